@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark: batched 4D-STEM plain multislice on one MI355X.
+
+Workload (the headline configuration): seeded amorphous-plus-crystal block
+64 x 64 x 300 Angstrom (73,728 atoms, Z in {6,14,29,79}), 512 x 512 complex64
+grid, dz = 2 Angstrom (150 slices), 200 kV, 25 mrad probe, 64 x 64 raster
+(4096 probe positions), float32 CBED per position plus BF (0-25 mrad) and ADF
+(40-200 mrad) detector integrals fused into the readout.
+
+A step is one full pass of the hot path over the batch: probe formation,
+150 fused slice steps and the CBED/detector readout for all 4096 positions,
+with slices, probe and propagator already resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line: probe-positions*slices/s, the roofline of the dominant
+kernel timed live with HIP events, and a CPU baseline (the op-for-op port of
+the reference CPU path) timed on this host's cores for a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "probe-positions·slices/sec at 512² FP32; learned vs plain at ≤1e-3 CBED rel-L2"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--probes", type=int, default=64, help="raster edge (probes = edge^2)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-probes", type=int, default=0, help="CPU sample size (0: 2 per core)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="run a 1 GiB device copy first (HBM counter calibration)")
+    p.add_argument("--kernel-reps", type=int, default=10)
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup(edge):
+    import torch
+
+    from namsa_amd import DEFAULT_PROBE, MSAGPU, structures
+
+    rec = structures.random_amorphous_crystal(seed=2)
+    sim = MSAGPU(200.0, 0.025, rec, sampling=np.array([512, 512]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    sim.build_probe(dict(DEFAULT_PROBE))
+    sim.generate_probe_positions(grid_steps=np.array([edge, edge]))
+    sim.build_propagator(1 / 3)
+    dets = np.stack([sim.detector_mask(0.0, 25e-3), sim.detector_mask(40e-3, 200e-3)])
+    torch.cuda.synchronize()
+    return sim, dets
+
+
+def cpu_baseline(sim, n_probes):
+    """The reference CPU path (oracle.cpu_port) on n_probes raster points."""
+    from oracle import cpu_port
+    from oracle import msa_oracle as O
+
+    kirk = np.load(ROOT / "namsa_amd" / "data" / "kirkland_params.npy")
+    sites, uniq = O.bin_sites(sim.supercell_xyz, sim.supercell_Z, sim.slice_t, sim.pix_size, sim.dims)
+    tables = np.stack([O.atomic_table(kirk[z - 1], sim.pix_size) for z in uniq])
+    V = O.splat_slices(tables, sites, 512, 512).astype(np.float32)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    n = n_probes or 2 * cores
+    idx = np.linspace(0, len(sim.probe_positions) - 1, n).astype(int)
+    # the CPU path takes positions as (x, y) (msa.py:225)
+    pos = sim.probe_positions[idx][:, ::-1]
+    t0 = time.perf_counter()
+    cpu_port.multislice_cpu(pos, V, sim.sigma, sim.kmax, sim.Lambda, sim.slice_t, sim.semi_ang,
+                            bandwidth=1 / 3, probe_kw=dict(apert_smooth=80.0), processes=min(cores, n))
+    dt = time.perf_counter() - t0
+    S = V.shape[0]
+    return {"value": n * S / dt, "unit": "probe-positions·slices/s", "cores": min(cores, n),
+            "kind": "port",
+            "sample": f"{n} raster points x {S} slices at 512^2 (oracle/cpu_port.py, numpy "
+                      f"{np.__version__}, fork Pool of {min(cores, n)}), {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X only: the multislice hot path shards by probe "
+              "position across GPUs without any exchange, and this build runs and reports the "
+              "single-GPU case (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import namsa_amd._lib as L
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB write")
+        del a, b
+
+    log("setup: structure, tables, slices, probe, raster")
+    sim, dets = setup(args.probes)
+    dev = sim.device
+    Ny = Nx = 512
+    P = int(sim.num_probes)
+    S = int(sim.num_slices)
+    plane = Ny * Nx
+    st = L.stream_handle()
+    pos = torch.from_numpy(np.ascontiguousarray(sim.probe_positions, dtype=np.float32)).to(dev)
+    psi = torch.empty((P, Ny, Nx), dtype=torch.complex64, device=dev)
+    cbed = torch.empty((P, Ny, Nx), dtype=torch.float32, device=dev)
+    det = torch.from_numpy(dets).to(dev)
+    lib = L.load()
+    det_ws = torch.empty(int(lib.namsa_det_workspace_doubles(P, Ny, Nx, 2)), dtype=torch.float64, device=dev)
+    det_out = torch.empty((P, 2), dtype=torch.float64, device=dev)
+    scale = np.float32(1.0 / sim.normalization)
+    kmax = np.float32(sim.kmax)
+
+    def step():
+        L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos), P, Ny, Nx, kmax, L.ptr(sim._t), S,
+               L.ptr(sim.pm_device), L.OUT_CBED, scale, L.ptr(psi), L.ptr(cbed), L.ptr(det), 2,
+               L.ptr(det_ws), L.ptr(det_out), st)
+
+    log(f"warmup {args.warmup} step(s): {P} probes x {S} slices")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} step(s)")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    value = P * S * args.steps / wall
+    log(f"wall {wall:.3f} s, events {gpu_ms / 1e3:.3f} s, {value:.4g} probe-slices/s")
+    sane = np.isfinite(det_out.cpu().numpy()).all()
+
+    # live per-kernel timing of the two fused passes on the stream they run on
+    L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), P, Ny, Nx, kmax, L.ptr(psi), st)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * args.kernel_reps)]
+    for r in range(args.kernel_reps):
+        ev[3 * r].record()
+        L.call("namsa_slice_rows", L.ptr(psi), P, Ny, Nx, L.ptr(sim._t[r % S]), st)
+        ev[3 * r + 1].record()
+        L.call("namsa_slice_cols", L.ptr(psi), P, Ny, Nx, L.ptr(sim.pm_device), st)
+        ev[3 * r + 2].record()
+    torch.cuda.synchronize()
+    row_ms = float(np.mean([ev[3 * r].elapsed_time(ev[3 * r + 1]) for r in range(args.kernel_reps)]))
+    col_ms = float(np.mean([ev[3 * r + 1].elapsed_time(ev[3 * r + 2]) for r in range(args.kernel_reps)]))
+    row_bytes = 16 * plane * P + 8 * plane      # wave R+W, slice read once
+    col_bytes = 16 * plane * P                  # wave R+W (propagator excluded)
+    kernels = {"row_pass": (row_ms, row_bytes), "col_pass": (col_ms, col_bytes)}
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    d_ms, d_bytes = kernels[dom]
+    achieved = d_bytes / (d_ms * 1e-3) / 1e9
+    step_gbs = (row_bytes + col_bytes) / ((row_ms + col_ms) * 1e-3) / 1e9
+    log(f"row pass {row_ms:.3f} ms ({row_bytes / row_ms / 1e6:.0f} GB/s), col pass {col_ms:.3f} ms "
+        f"({col_bytes / col_ms / 1e6:.0f} GB/s), slice step {step_gbs:.0f} GB/s")
+
+    traffic = None
+    tf = ROOT / "profiles" / "hbm_traffic_config2_512.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get(dom)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(sim, args.cpu_probes)
+        log(f"cpu baseline {cpu['value']:.4g} {cpu['unit']} ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "probe-positions·slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded structure, generated on the fly)",
+        "config": {"workload": "batched 4D-STEM plain multislice: 64x64x300 A amorphous+crystal "
+                               "(73,728 atoms, Z in {6,14,29,79}), 512x512 complex64, dz=2 A "
+                               "(150 slices), 200 kV, 25 mrad, 64x64 raster, CBED + BF/ADF",
+                   "grid": [Ny, Nx], "slices": S, "probe_positions": P, "batch": P,
+                   "method": "plain"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": d_bytes, "avg_launch_ms": d_ms,
+                     "slice_step_GBs": step_gbs, "row_ms": row_ms, "col_ms": col_ms},
+        "cpu_baseline": cpu,
+        "finite_outputs": bool(sane),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * namsa_hip.h -- C ABI of the MI355X (gfx950) multislice library libnamsa_hip.so.
+ *
+ * Every entry point replaces a piece of the reference nlaanait/namsa MSAGPU
+ * path, which launched pycuda kernels (namsa/cuda_kernels.py:8-56) and
+ * scikit-cuda cuFFT plans from Python.  Conventions:
+ *   - all array pointers are DEVICE pointers owned by the caller (e.g. torch
+ *     tensors' data_ptr()); complex arrays are interleaved complex64
+ *     (float re, float im), C order;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream); every
+ *     call only enqueues work on it and never synchronises the host;
+ *   - no entry point allocates device memory; scratch space is passed in;
+ *   - every call returns 0 on success or a negative status; the message of
+ *     the last failure on the calling thread is namsa_last_error();
+ *   - grid sizes Ny, Nx must be supported FFT lengths (namsa_fft_supported).
+ */
+#ifndef NAMSA_HIP_H
+#define NAMSA_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NAMSA_ABI_VERSION 1
+
+#define NAMSA_OK 0
+#define NAMSA_ERR_ARG (-1)
+#define NAMSA_ERR_UNSUPPORTED (-2)
+#define NAMSA_ERR_HIP (-3)
+
+/* Output selector of namsa_multislice. */
+#define NAMSA_OUT_CBED 0     /* transmit=True: float32 CBED intensities   */
+#define NAMSA_OUT_INCIDENT 1 /* transmit=False: complex incident probes   */
+#define NAMSA_OUT_EXIT 2     /* complex real-space exit waves              */
+
+int namsa_abi_version(void);
+const char* namsa_last_error(void);
+
+/* Select the device (replaces setup_device, namsa/msa.py:535-553). */
+int namsa_init(int device);
+
+/* 1 if n is a compiled transform length, else 0. */
+int namsa_fft_supported(int n);
+
+/*
+ * Kirkland projected-potential tables, one Py x Px float32 table per element
+ * (replaces MSA.calc_atomic_potentials / _get_potential, namsa/msa.py:137-146,
+ * :180-196, and bin_2d_array, namsa/utils.py:101-108).
+ *   params:  [nZ][4][3] float64 rows a, b, c, d (kirkland_params[Z-1])
+ *   tables:  [nZ][Py][Px] float32 output
+ * Subsamples sit on linspace(-range/2, range/2, oversample*P) per axis, the
+ * table is their oversample x oversample block mean shifted so its minimum is 0.
+ */
+int namsa_atomic_tables(const double* params, int nZ, int Py, int Px, double range,
+                        int oversample, float* tables, void* stream);
+
+/*
+ * Transmission slices t = exp(i sigma V) (replaces BuildScatteringPotential,
+ * namsa/potential_kernels.cu:5-60, launched at namsa/msa.py:626-639).
+ *   tables:        [nZ][Py][Px] float32
+ *   sites:         [A][3] int32 (table index, y pixel, x pixel), grouped by slice
+ *   slice_offsets: [S+1] int32, sites of slice s are [offsets[s], offsets[s+1])
+ *   t_out:         [S][Ny][Nx] complex64 output
+ * Site footprint rows are rintf(y - Py/2) <= row < rintf(y + Py/2), clipped to
+ * the grid and to the table; sites with a negative table index are skipped.
+ * NaN components of t are replaced by 1.
+ */
+int namsa_build_slices(const float* tables, int nZ, int Py, int Px, const int* sites,
+                       const int* slice_offsets, int S, int Ny, int Nx, float sigma,
+                       void* t_out, void* stream);
+
+/*
+ * Probe in k-space psi_k = A(k) exp(-i chi(k)) on the linspace k-grid
+ * (replaces soft_aperture / hard_aperture / spherical_phase_error /
+ * mult_wise_c2d_re2d, namsa/probe_kernels.cu:180-222 and :126-134, launched at
+ * namsa/msa.py:711-715).
+ *   smooth != 0: A = 1/(1+exp(-2*steepness*(k_semi-k))); else A = [k < k_semi].
+ *   psi_k: [Ny][Nx] complex64 output; apert: [Ny][Nx] float32 output or NULL.
+ */
+int namsa_probe_k(int Ny, int Nx, float kmax, float k_semi, float lambda, float C1, float C3,
+                  float C5, int smooth, float steepness, void* psi_k, float* apert,
+                  void* stream);
+
+/*
+ * Fresnel propagator times bandwidth mask (replaces propagator, hard_aperture
+ * and mult_wise_c2d_re2d, namsa/probe_kernels.cu:167-194, at
+ * namsa/msa.py:833-839).
+ *   pm:   [Ny][Nx] complex64 output exp(-i pi k^2 lambda dz) * [k < k_cut]
+ *   mask: [Ny][Nx] float32 output or NULL
+ */
+int namsa_propagator(int Ny, int Nx, float kmax, float k_cut, float lambda, float dz,
+                     void* pm, float* mask, void* stream);
+
+/*
+ * Batched unnormalised 2D C2C transform in place (replaces the scikit-cuda
+ * cuFFT plan + cufftExecC2C calls, namsa/msa.py:721-722, :880, :962, :976,
+ * :981, :987).  inverse = 0: exp(-2 pi i ...), inverse = 1: exp(+2 pi i ...).
+ *   data: [B][Ny][Nx] complex64
+ */
+int namsa_fft2d(void* data, int B, int Ny, int Nx, int inverse, void* stream);
+
+/* sum |x|^2 over n complex64 values into *out (one float64 on the device);
+ * the normalization of build_probe, namsa/msa.py:727. */
+int namsa_norm2(const void* data, long long n, double* out, void* stream);
+
+/*
+ * Detector integrals of stored CBEDs (replaces the host sum of
+ * MSA.integrate_cbed, namsa/msa.py:436-439).
+ *   cbed: [B][Ny][Nx] float32; det: [nDet][Ny][Nx] float32;
+ *   out:  [B][nDet] float64 = sum over pixels of cbed * det, in float64.
+ */
+int namsa_integrate(const float* cbed, int B, int Ny, int Nx, const float* det, int nDet,
+                    double* out, void* stream);
+
+/*
+ * Number of float64 scratch values namsa_multislice needs for the detector
+ * partial sums of B probes and nDet detectors.
+ */
+size_t namsa_det_workspace_doubles(int B, int Ny, int Nx, int nDet);
+
+/*
+ * Building blocks of namsa_multislice, one launch each, for callers that
+ * schedule slices themselves (graph capture, per-pass timing).  The wave
+ * state between them is internal: after namsa_probe_init and after every
+ * namsa_slice_cols it holds C^-1 of the spectrum of the current wave; after
+ * namsa_slice_rows it holds the row transform of t_i * psi_i / N^2.
+ *   namsa_probe_init  probe stack (build_probes_stack + IFFT + fftshift_2d_stack,
+ *                     namsa/probe_kernels.cu:239-304, msa.py:959-964)
+ *   namsa_slice_rows  first half of one slice (msa.py:973-976): row pass
+ *   namsa_slice_cols  second half of one slice (msa.py:976-981): column pass
+ * A slice i is namsa_slice_rows(t + i*Ny*Nx) followed by namsa_slice_cols.
+ */
+int namsa_probe_init(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                     void* psi_ws, void* stream);
+int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, void* stream);
+int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, void* stream);
+
+/*
+ * Batched multislice, fused (replaces MSAGPU.__propagate_beams,
+ * namsa/msa.py:926-992: build_probes_stack, fftshift_2d_stack,
+ * mult_wise_c3d_c3d_ind, mult_wise_c3d_c2d, mod_square_stack and the four
+ * cuFFT executions per slice; and the detector sums of integrate_cbed,
+ * namsa/msa.py:426-439).
+ *   psi_k:   [Ny][Nx] complex64 probe in k-space
+ *   pos_yx:  [B][2] float32 probe positions (y, x) in Angstrom
+ *   t:       [S][Ny][Nx] complex64 transmission slices, applied in ascending order
+ *   pm:      [Ny][Nx] complex64 propagator * mask
+ *   mode:    NAMSA_OUT_CBED     cbed[B][Ny][Nx] = |sqrt(N^2) FFT(psi_S/N^2)|^2 * out_scale
+ *                               (out_scale = 1/normalization), plus
+ *                               det_out[B][nDet] = sum(cbed * det[d]) if nDet > 0
+ *            NAMSA_OUT_INCIDENT psi_ws[B][Ny][Nx] = incident probes * out_scale
+ *            NAMSA_OUT_EXIT     psi_ws[B][Ny][Nx] = real-space exit waves * out_scale
+ *   psi_ws:  [B][Ny][Nx] complex64 wave state (scratch or output)
+ *   det:     [nDet][Ny][Nx] float32 detector weights (may be NULL if nDet == 0)
+ *   det_ws:  namsa_det_workspace_doubles(B, Ny, Nx, nDet) float64 scratch values
+ *   det_out: [B][nDet] float64
+ */
+int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                     const void* t, int S, const void* pm, int mode, float out_scale,
+                     void* psi_ws, float* cbed, const float* det, int nDet, double* det_ws,
+                     double* det_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NAMSA_HIP_H */

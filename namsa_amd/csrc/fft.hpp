@@ -1,0 +1,271 @@
+// Register-resident Stockham FFT engine for gfx950.
+//
+// A length-N transform is computed by T = N/E cooperating threads, each holding
+// E elements in registers in the "natural distributed" layout: thread t holds
+// element t + T*m in register m.  A stage of radix R (R | E) performs E/R
+// radix-R butterflies per thread on registers {q + r*E/R}; its outputs go to
+// LDS at Stockham autosort positions and are read back in the natural layout
+// for the next stage.  The last stage writes its outputs back to the very
+// registers it read, so a transform starts and ends in the natural layout and
+// a pointwise multiply can sit between an inverse and a forward transform with
+// no data movement.  Twiddles depend only on (stage, t) and are computed once
+// per thread in double precision (sincospi) and kept in registers.
+//
+// Sign convention: forward = exp(-2 pi i jk/N), inverse = exp(+2 pi i jk/N),
+// both unnormalised (the cuFFT C2C convention the reference calls into,
+// namsa/msa.py:962,976,981,987).
+#pragma once
+#include "common.hpp"
+
+namespace namsa {
+
+// ------------------------------------------------------------------ radix-R DFTs
+template <int R, bool INV>
+struct Dft;
+
+template <bool INV>
+struct Dft<2, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <bool INV>
+struct Dft<3, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float s = 0.86602540378443865f;  // sqrt(3)/2
+    float2 t1 = cadd(a[1], a[2]);
+    float2 t2 = cscale(csub(a[1], a[2]), s);
+    float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+    a[0] = cadd(a[0], t1);
+    float2 r = mul_mi<INV>(t2);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  }
+};
+
+template <bool INV>
+struct Dft<4, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    float2 t0 = cadd(a[0], a[2]);
+    float2 t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]);
+    float2 t3 = mul_mi<INV>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <bool INV>
+struct Dft<8, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    const float c = 0.70710678118654752f;
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, INV>::run(e);
+    Dft<4, INV>::run(o);
+    // o[k] *= exp(-+ 2 pi i k / 8)
+    if (INV) {
+      o[1] = make_float2(c * (o[1].x - o[1].y), c * (o[1].x + o[1].y));
+      o[3] = make_float2(-c * (o[3].x + o[3].y), c * (o[3].x - o[3].y));
+    } else {
+      o[1] = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
+      o[3] = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+    }
+    o[2] = mul_mi<INV>(o[2]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = cadd(e[k], o[k]);
+      a[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+template <bool INV>
+struct Dft<16, INV> {
+  static __device__ __forceinline__ void run(float2* a) {
+    // cos / sin of k*pi/8, k = 0..7
+    const float C[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                        0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
+    const float S[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+                        1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+    float2 e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      e[k] = a[2 * k];
+      o[k] = a[2 * k + 1];
+    }
+    Dft<8, INV>::run(e);
+    Dft<8, INV>::run(o);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      float2 w = make_float2(C[k], INV ? S[k] : -S[k]);
+      if (k == 4)
+        o[k] = mul_mi<INV>(o[k]);
+      else
+        o[k] = cmul(o[k], w);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[k] = cadd(e[k], o[k]);
+      a[k + 8] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// ------------------------------------------------------------------ schedules
+template <int... Rs>
+struct Radices {};
+
+template <int... Rs>
+struct Prod;
+template <>
+struct Prod<> {
+  static constexpr int value = 1;
+};
+template <int R, int... Rs>
+struct Prod<R, Rs...> {
+  static constexpr int value = R * Prod<Rs...>::value;
+};
+
+// Twiddle registers a thread needs for stages with NS > 1.
+template <int E, int NS, int... Rs>
+struct TwCount;
+template <int E, int NS>
+struct TwCount<E, NS> {
+  static constexpr int value = 0;
+};
+template <int E, int NS, int R, int... Rs>
+struct TwCount<E, NS, R, Rs...> {
+  static constexpr int value = (NS > 1 ? (E / R) * (R - 1) : 0) + TwCount<E, NS * R, Rs...>::value;
+};
+
+// Synchronisation between the LDS write and read phases of one exchange.
+struct WaveSync {  // all threads of one transform live in one wavefront
+  static __device__ __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+struct BlockSync {
+  static __device__ __forceinline__ void sync() { __syncthreads(); }
+};
+
+// LDS accessors.  RowLds: one transform per contiguous (padded) region.
+// ColLds: C interleaved transforms, element (i, c) at i*C + c (padded).
+struct RowLds {
+  float2* base;
+  __device__ __forceinline__ float2& at(int i) const { return base[i + (i >> 4)]; }
+  static constexpr int padded(int n) { return n + n / 16; }
+};
+template <int C>
+struct ColLds {
+  float2* base;
+  int c;
+  __device__ __forceinline__ float2& at(int i) const { return base[i * C + c + ((i * C) >> 5)]; }
+  static constexpr int padded(int n) { return n * C + (n * C) / 32; }
+};
+
+template <int N, int E, class RL>
+struct Fft;
+
+template <int N, int E, int... Rs>
+struct Fft<N, E, Radices<Rs...>> {
+  static_assert(Prod<Rs...>::value == N, "radix schedule must multiply to N");
+  static constexpr int T = N / E;
+  static constexpr int NTW = TwCount<E, 1, Rs...>::value;
+  static constexpr int NTW_ALLOC = NTW > 0 ? NTW : 1;
+  static constexpr int NSTAGES = sizeof...(Rs);
+
+  // ---------------- twiddles (forward sign; inverse uses the conjugate)
+  template <int NS, int OFF>
+  static __device__ __forceinline__ void tw_stages(int, float2 (&)[NTW_ALLOC]) {}
+
+  template <int NS, int OFF, int R, int... Rest>
+  static __device__ __forceinline__ void tw_stages(int t, float2 (&tw)[NTW_ALLOC]) {
+    constexpr int Q = E / R;
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k = (t + T * q) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          double s, c;
+          sincospi(-2.0 * double(r * k) / double(NS * R), &s, &c);
+          tw[OFF + q * (R - 1) + (r - 1)] = make_float2(float(c), float(s));
+        }
+      }
+      tw_stages<NS * R, OFF + Q * (R - 1), Rest...>(t, tw);
+    } else {
+      tw_stages<NS * R, OFF, Rest...>(t, tw);
+    }
+  }
+
+  static __device__ __forceinline__ void twiddles(int t, float2 (&tw)[NTW_ALLOC]) {
+    tw_stages<1, 0, Rs...>(t, tw);
+  }
+
+  // ---------------- one stage
+  template <bool INV, int R, int NS, int OFF, bool LAST, class Lds, class Sync>
+  static __device__ __forceinline__ void stage(float2 (&v)[E], const Lds& lds, int t,
+                                               const float2 (&tw)[NTW_ALLOC]) {
+    constexpr int Q = E / R;
+    static_assert(E % R == 0, "radix must divide the per-thread element count");
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      float2 a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const float2 w = tw[OFF + q * (R - 1) + (r - 1)];
+          a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+        }
+      }
+      Dft<R, INV>::run(a);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q + r * Q] = a[r];
+      } else {
+        const int j = t + T * q;
+        const int k = j % NS;
+        const int dst = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds.at(dst + r * NS) = a[r];
+      }
+    }
+    if constexpr (!LAST) {
+      Sync::sync();
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = lds.at(t + T * m);
+      Sync::sync();
+    }
+  }
+
+  template <bool INV, int NS, int OFF, class Lds, class Sync>
+  static __device__ __forceinline__ void stages(float2 (&)[E], const Lds&, int,
+                                                const float2 (&)[NTW_ALLOC]) {}
+
+  template <bool INV, int NS, int OFF, class Lds, class Sync, int R, int... Rest>
+  static __device__ __forceinline__ void stages(float2 (&v)[E], const Lds& lds, int t,
+                                                const float2 (&tw)[NTW_ALLOC]) {
+    stage<INV, R, NS, OFF, sizeof...(Rest) == 0, Lds, Sync>(v, lds, t, tw);
+    constexpr int NEXT_OFF = OFF + (NS > 1 ? (E / R) * (R - 1) : 0);
+    stages<INV, NS * R, NEXT_OFF, Lds, Sync, Rest...>(v, lds, t, tw);
+  }
+
+  // Full transform, natural layout in and out.
+  template <bool INV, class Sync, class Lds>
+  static __device__ __forceinline__ void run(float2 (&v)[E], const Lds& lds, int t,
+                                             const float2 (&tw)[NTW_ALLOC]) {
+    stages<INV, 1, 0, Lds, Sync, Rs...>(v, lds, t, tw);
+  }
+};
+
+}  // namespace namsa

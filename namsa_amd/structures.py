@@ -1,0 +1,143 @@
+"""Seeded synthetic structures in the reference's supercell record layout.
+
+The reference feeds MSA a structured array with fields atomic_number, x, y,
+z, occ, DW (namsa/supercell.py:446-450, namsa/msa.py:76-99).  The generators
+here produce the same record layout for the benchmark workloads:
+
+  si111_supercell        diamond-cubic Si along [111], orthogonal cell
+                         [1-10] x [11-2] x [111], tiled laterally and cropped
+                         to a thickness
+  random_amorphous_crystal  Z drawn from a set, a jittered simple-cubic
+                         lattice in the lower half and random sequential
+                         addition with a minimum spacing in the upper half
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SUPERCELL_DTYPE = [("atom_type", "U2"), ("atomic_number", "i4"), ("x", "f4"), ("y", "f4"),
+                   ("z", "f4"), ("occ", "f4"), ("DW", "f4")]
+DW_DEFAULT = 0.07  # Angstrom, the value SupercellBuilder always uses (supercell.py:105-109)
+
+_SYMBOLS = {1: "H", 3: "Li", 6: "C", 7: "N", 8: "O", 13: "Al", 14: "Si", 29: "Cu", 31: "Ga",
+            33: "As", 79: "Au", 83: "Bi"}
+
+
+def make_records(xyz, Z, dw=DW_DEFAULT):
+    """Pack positions (A,3) and atomic numbers (A,) into supercell records."""
+    xyz = np.asarray(xyz, dtype=np.float32).reshape(-1, 3)
+    Z = np.asarray(Z, dtype=np.int32).reshape(-1)
+    rec = np.zeros(len(Z), dtype=SUPERCELL_DTYPE)
+    rec["atomic_number"] = Z
+    rec["atom_type"] = [_SYMBOLS.get(int(z), "X") for z in Z]
+    rec["x"], rec["y"], rec["z"] = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    rec["occ"] = 1.0
+    rec["DW"] = dw
+    return rec
+
+
+def si111_supercell(a=5.431, reps=(5, 3), thickness=200.0):
+    """Diamond Si with [111] along z, lateral reps of the [1-10] x [11-2] cell.
+
+    The orthogonal cell spans a*(1,-1,0), a*(1,1,-2), a*(1,1,1) (6 cubic cells,
+    48 atoms); it is tiled reps[0] x reps[1] laterally and enough times along
+    z, then cropped to 0 <= z < thickness.
+    """
+    basis_fcc = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    basis = np.concatenate([basis_fcc, basis_fcc + 0.25])  # diamond: 8 atoms / cubic cell
+    u = np.array([1, -1, 0]) / np.sqrt(2)
+    v = np.array([1, 1, -2]) / np.sqrt(6)
+    w = np.array([1, 1, 1]) / np.sqrt(3)
+    L = np.array([np.sqrt(2), np.sqrt(6), np.sqrt(3)]) * a
+    rng = np.arange(-3, 4)
+    cells = np.array(np.meshgrid(rng, rng, rng, indexing="ij")).reshape(3, -1).T
+    pts = (cells[:, None, :] + basis[None]).reshape(-1, 3) * a
+    rot = np.stack([pts @ u, pts @ v, pts @ w], axis=1)
+    tol = 1e-6
+    keep = np.all((rot >= -tol) & (rot < L - tol), axis=1)
+    cell = np.unique(np.round(rot[keep], 6), axis=0)
+    assert len(cell) == 48, len(cell)
+    nz = int(np.ceil(thickness / L[2])) + 1
+    shifts = np.array(np.meshgrid(np.arange(reps[0]), np.arange(reps[1]), np.arange(nz),
+                                  indexing="ij")).reshape(3, -1).T * L
+    xyz = (shifts[:, None, :] + cell[None]).reshape(-1, 3)
+    xyz = xyz[xyz[:, 2] < thickness]
+    order = np.lexsort((xyz[:, 0], xyz[:, 1], xyz[:, 2]))
+    return make_records(xyz[order], np.full(len(order), 14))
+
+
+def _rsa(rng, n, lo, hi, dmin, existing):
+    """Random sequential addition of n points in the box [lo, hi) at spacing >= dmin."""
+    cell = dmin
+    dims = np.ceil((hi - lo) / cell).astype(int) + 1
+    grid = {}
+
+    def key(p):
+        return tuple(((p - lo) // cell).astype(int))
+
+    for p in existing:
+        grid.setdefault(key(p), []).append(p)
+    out = []
+    attempts = 0
+    while len(out) < n:
+        attempts += 1
+        if attempts > 200 * n + 10000:
+            raise RuntimeError("random sequential addition did not converge")
+        p = lo + rng.random(3) * (hi - lo)
+        k = key(p)
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dzz in (-1, 0, 1):
+                    for q in grid.get((k[0] + dx, k[1] + dy, k[2] + dzz), ()):
+                        if np.sum((p - q) ** 2) < dmin * dmin:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid.setdefault(k, []).append(p)
+            out.append(p)
+    del dims
+    return np.array(out).reshape(-1, 3)
+
+
+def random_amorphous_crystal(seed=2, lateral=64.0, thickness=300.0, density=0.06,
+                             zset=(6, 14, 29, 79), dmin=1.2, jitter=0.1):
+    """Seeded amorphous-plus-crystal block.
+
+    Lower half (z < thickness/2): simple-cubic-like lattice with
+    n_xy x n_xy x n_z sites at the target density, each jittered uniformly by
+    +-jitter Angstrom.  Upper half: random sequential addition at the same
+    density with spacing >= dmin (checked against the lattice half too).  Z is
+    uniform over zset.  One upper-half atom is pinned at z = thickness so the
+    extent is exactly `thickness` (the slice count is floor(extent/dz)).
+    """
+    rng = np.random.default_rng(seed)
+    half = thickness / 2
+    n_half = int(round(lateral * lateral * half * density))
+    est = (n_half / (half / lateral)) ** (1 / 3)
+    exact = [n for n in range(max(1, int(est) - 4), int(est) + 5) if n_half % (n * n) == 0]
+    n_xy = min(exact, key=lambda n: abs(n - est)) if exact else int(round(est))
+    n_z = int(round(n_half / (n_xy * n_xy)))
+    ax = (np.arange(n_xy) + 0.5) * lateral / n_xy
+    az = np.arange(n_z) * half / n_z
+    X, Y, Zc = np.meshgrid(ax, ax, az, indexing="ij")
+    lat = np.stack([X.ravel(), Y.ravel(), Zc.ravel()], 1)
+    lat[:, :2] += rng.uniform(-jitter, jitter, size=(len(lat), 2))
+    lat[:, 2] += rng.uniform(0, jitter, size=len(lat))
+    lo = np.array([0.0, 0.0, half])
+    hi = np.array([lateral, lateral, thickness])
+    border = lat[lat[:, 2] > half - dmin]
+    top = _rsa(rng, n_half, lo, hi, dmin, border)
+    xyz = np.concatenate([lat, top])
+    xyz[:, 2] -= xyz[:, 2].min()
+    # pin the extent: the topmost atom moves up to z = thickness (only away
+    # from every other atom, so the spacing rule still holds)
+    xyz[int(np.argmax(xyz[:, 2])), 2] = thickness
+    Z = rng.choice(np.asarray(zset), size=len(xyz))
+    return make_records(xyz, Z)

@@ -1,0 +1,52 @@
+"""The C ABI library loads and exports every symbol include/namsa_hip.h declares.
+CPU only: no compute call is made; argument checks that fail before any HIP
+call are exercised."""
+import ctypes
+import re
+import subprocess
+
+import pytest
+
+import namsa_amd._lib as L
+
+
+def test_library_loads_with_expected_abi():
+    lib = L.load()
+    assert lib.namsa_abi_version() == L.ABI_VERSION
+
+
+def test_every_declared_symbol_is_exported():
+    declared = L.declared_functions()
+    assert len(declared) >= 12
+    out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (namsa_\w+)", out))
+    assert set(declared) <= exported, set(declared) - exported
+    assert set(declared) == set(L.SIGNATURES)
+
+
+def test_abi_version_macro_matches():
+    text = L.HEADER_PATH.read_text()
+    assert int(re.search(r"#define NAMSA_ABI_VERSION (\d+)", text).group(1)) == L.ABI_VERSION
+
+
+def test_supported_sizes():
+    lib = L.load()
+    for n in (128, 192, 256, 384, 512, 768, 1024, 2048):
+        assert lib.namsa_fft_supported(n) == 1
+    for n in (0, 100, 500, 4096):
+        assert lib.namsa_fft_supported(n) == 0
+
+
+def test_argument_errors_report_without_gpu():
+    lib = L.load()
+    rc = lib.namsa_fft2d(None, 1, 512, 512, 0, None)
+    assert rc == -1 and b"null data" in lib.namsa_last_error()
+    rc = lib.namsa_fft2d(ctypes.c_void_p(16), 1, 500, 512, 0, None)
+    assert rc == -2 and b"500" in lib.namsa_last_error()
+    rc = lib.namsa_multislice(None, None, 0, 512, 512, 8.0, None, 0, None, 0, 1.0, None, None, None,
+                              0, None, None, None)
+    assert rc == -1
+    with pytest.raises(L.NamsaError):
+        L.call("namsa_norm2", None, 0, None, None)
+    assert lib.namsa_det_workspace_doubles(4, 512, 512, 2) == 4 * (512 // 16) * 2

@@ -1,0 +1,290 @@
+"""HIP path vs the FP64 oracle on the same seeded inputs (MI355X).
+
+Tolerances (stated per test): transforms and pointwise kernels to FP32
+rounding; CBED intensities to 1e-4 relative L2 (the plain-method parity bar);
+discrete decisions (bandwidth mask, aperture) bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import namsa_amd._lib as L
+from namsa_amd import MSAGPU, structures
+from oracle import msa_oracle as O
+from tests.parity_util import (KIRK, oracle_pm, oracle_probe, oracle_slices, random_block,
+                               rel_l2)
+
+pytestmark = pytest.mark.gpu
+SIZES = [128, 192, 256, 384, 512, 768, 1024, 2048]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _dev_c64(torch, arr, dev):
+    return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex64)).to(dev)
+
+
+@pytest.mark.parametrize("N", SIZES)
+def test_fft2d_matches_numpy(gpu, N):
+    import torch
+
+    _say(f"fft2d N={N}")
+    rng = np.random.default_rng(N)
+    B = 3 if N <= 1024 else 2
+    x = (rng.standard_normal((B, N, N)) + 1j * rng.standard_normal((B, N, N))).astype(np.complex64)
+    for inverse in (0, 1):
+        d = _dev_c64(torch, x, gpu)
+        L.call("namsa_fft2d", L.ptr(d), B, N, N, inverse, L.stream_handle())
+        got = d.cpu().numpy()
+        x64 = x.astype(np.complex128)
+        ref = np.fft.ifft2(x64) * N * N if inverse else np.fft.fft2(x64)
+        err = rel_l2(got, ref)
+        _say(f"  inverse={inverse} relL2={err:.2e}")
+        assert err < 3e-6   # FP32 rounding of an N^2-point transform
+
+
+def test_fft2d_rectangular(gpu):
+    import torch
+
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal((2, 256, 512)) + 1j * rng.standard_normal((2, 256, 512))).astype(np.complex64)
+    d = _dev_c64(torch, x, gpu)
+    L.call("namsa_fft2d", L.ptr(d), 2, 256, 512, 0, L.stream_handle())
+    assert rel_l2(d.cpu().numpy(), np.fft.fft2(x.astype(np.complex128))) < 3e-6
+
+
+@pytest.mark.parametrize("pix", [(0.125, 0.125), (0.15, 0.148), (0.0625, 0.0625)])
+def test_atomic_tables_match_oracle(gpu, pix):
+    import torch
+
+    _say(f"atomic tables pix={pix}")
+    Zs = np.array([3, 6, 14, 29, 79, 83])
+    Py, Px = O.table_shape(np.asarray(pix))
+    params = torch.from_numpy(np.ascontiguousarray(KIRK[Zs - 1])).to(gpu)
+    out = torch.empty((len(Zs), Py, Px), dtype=torch.float32, device=gpu)
+    L.call("namsa_atomic_tables", L.ptr(params), len(Zs), Py, Px, 8.0, 2, L.ptr(out), L.stream_handle())
+    got = out.cpu().numpy()
+    for i, Z in enumerate(Zs):
+        ref = O.atomic_table(KIRK[Z - 1], np.asarray(pix))
+        # float32 storage of FP64 values: a few ulp of the table maximum
+        np.testing.assert_allclose(got[i], ref, rtol=0, atol=4e-7 * ref.max())
+
+
+def test_slices_match_oracle_small(gpu):
+    _say("slices small 256^2")
+    rec = random_block(11, 32.0, 12.0, 400)
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    t_ref, tables = oracle_slices(sim, 2.0)
+    for z, tab in zip(sim.unique_Z, tables):
+        np.testing.assert_allclose(sim.cached_pots[int(z)], tab, rtol=0, atol=4e-7 * tab.max())
+    got = sim.potential_slices
+    assert got.shape == t_ref.shape
+    # phase error = sigma * (float32 sum error of V) + sincosf rounding
+    assert np.abs(got - t_ref).max() < 2e-5
+
+
+def test_slices_odd_table_and_edges(gpu):
+    _say("slices odd table 192^2")
+    rec = random_block(12, 28.7, 6.0, 150)
+    sim = MSAGPU(100, 0.02, rec, sampling=np.array([192, 192]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    assert sim.table_shape == (53, 53)   # floor(8 / (28.7/192)): odd, rint half-to-even placement
+    sim.build_potential_slices(1.5)
+    t_ref, _ = oracle_slices(sim, 1.5)
+    assert np.abs(sim.potential_slices - t_ref).max() < 2e-5
+
+
+def test_probe_and_propagator_match_oracle(gpu):
+    _say("probe / propagator 512^2")
+    rec = structures.random_amorphous_crystal(seed=2)
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([512, 512]), debye_waller=False)
+    sim.setup_device(0)
+    sim.slice_t = 2.0
+    for smooth in (True, False):
+        sim.build_probe({**sim_probe(), "smooth_apert": smooth})
+        psi_k, A = oracle_probe(sim, smooth=smooth)
+        np.testing.assert_allclose(sim.apert, A, rtol=0, atol=2e-7)
+        np.testing.assert_allclose(sim.psi_k, psi_k, rtol=0, atol=2e-7)
+        psi, norm = O.probe_real(psi_k)
+        assert sim.normalization == pytest.approx(norm, rel=1e-6)
+        assert rel_l2(sim.psi, psi / np.sqrt(norm)) < 2e-6
+    sim.build_propagator(1 / 3)
+    pm = oracle_pm(sim, 2.0, 1 / 3)
+    np.testing.assert_array_equal(sim.mask_device.cpu().numpy(), np.abs(pm) > 0)   # bit-exact mask
+    np.testing.assert_allclose(sim.pm_device.cpu().numpy(), pm, rtol=0, atol=2e-7)
+
+
+def test_probe_with_aberrations(gpu):
+    rec = random_block(3, 40.0, 10.0, 100)
+    sim = MSAGPU(300, 0.02, rec, sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    pd = {**sim_probe(), "aberration_dict": {"C1": 50.0, "C3": 1.2e7, "C5": 3e9}, "scherzer": False}
+    sim.build_probe(pd)
+    psi_k, _ = oracle_probe(sim, C=(50.0, 1.2e7, 3e9))
+    np.testing.assert_allclose(sim.psi_k, psi_k, rtol=0, atol=3e-6)
+
+
+def sim_probe():
+    from namsa_amd import DEFAULT_PROBE
+
+    return dict(DEFAULT_PROBE)
+
+
+def _run_sim(sim, dz, positions, bandwidth=1 / 3, **kw):
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(dz)
+    sim.build_probe(sim_probe())
+    sim.probe_positions = np.asarray(positions, dtype=np.float64)
+    sim.num_probes = np.int32(len(positions))
+    sim.grid_steps = np.array([len(positions), 1])
+    sim.plan_simulation()
+    sim.multislice(bandwidth=bandwidth, **kw)
+
+
+@pytest.mark.parametrize("N,S_thick", [(256, 20.0), (512, 16.0), (384, 12.0)])
+def test_cbed_parity(gpu, N, S_thick):
+    _say(f"cbed parity N={N}")
+    rec = random_block(20 + N, 24.0, S_thick, int(24 * 24 * S_thick * 0.06))
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([N, N]), debye_waller=False)
+    pos = np.array([[0.0, 0.0], [3.3, -2.1], [-5.25, 7.7], [0.01, 0.02], [6.0, 6.0]])
+    _run_sim(sim, 2.0, pos)
+    t_ref, _ = oracle_slices(sim, 2.0)
+    psi_k, _ = oracle_probe(sim)
+    cb_ref = O.cbed(psi_k, pos.astype(np.float32), np.float32(sim.kmax), t_ref, oracle_pm(sim, 2.0, 1 / 3))
+    errs = [rel_l2(sim.probes[i], cb_ref[i]) for i in range(len(pos))]
+    _say(f"  S={sim.num_slices} CBED relL2 per probe {['%.2e' % e for e in errs]}")
+    assert max(errs) < 1e-4
+
+
+def test_exit_wave_and_incident_parity(gpu):
+    _say("exit wave / incident 256^2")
+    rec = random_block(7, 30.0, 16.0, 800)
+    sim = MSAGPU(100, 0.02, rec, sampling=np.array([256, 256]), debye_waller=False)
+    pos = np.array([[1.0, -1.0], [-4.5, 2.25], [0.0, 8.0]])
+    _run_sim(sim, 2.0, pos, output="exit_wave")
+    t_ref, _ = oracle_slices(sim, 2.0)
+    psi_k, _ = oracle_probe(sim)
+    psi0 = O.probe_stack(psi_k, pos.astype(np.float32), np.float32(sim.kmax))
+    _, norm = O.probe_real(psi_k)
+    ex = O.multislice(psi0, t_ref, oracle_pm(sim, 2.0, 1 / 3)) / np.sqrt(norm)
+    assert rel_l2(sim.probes, ex) < 1e-4
+    sim.multislice(transmit=False)
+    assert rel_l2(sim.probes, psi0 / np.sqrt(norm)) < 2e-6
+
+
+def test_fused_detectors_and_integrate(gpu):
+    _say("detectors 256^2")
+    rec = random_block(8, 32.0, 10.0, 600)
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    sim.build_probe(sim_probe())
+    sim.generate_probe_positions(grid_steps=np.array([3, 2]))
+    sim.plan_simulation()
+    dets = np.stack([sim.detector_mask(0.0, 25e-3), sim.detector_mask(40e-3, 0.9 * sim.max_ang),
+                     sim.detector_mask(10e-3, 60e-3, mode="physical")])
+    sim.multislice(detectors=dets)
+    cb = sim.probes.astype(np.float64)
+    ref = np.einsum("pyx,dyx->pd", cb, dets.astype(np.float64))
+    np.testing.assert_allclose(sim.detector_signals, ref, rtol=1e-6)
+    sig, det = sim.integrate_cbed(detector_params={"inner_angle": 0.0, "outer_angle": 25e-3})
+    np.testing.assert_allclose(sig.ravel(), ref[:, 0], rtol=1e-6)
+    assert sig.shape == (3, 2)
+
+
+def test_unitarity_full_mask_config2_size(gpu):
+    """Size-independent property at full size: with the band limit off, every
+    slice is unitary, so sum(CBED) = 1 for each of B probes after 150 slices."""
+    _say("unitarity 512^2 S=150")
+    rec = structures.random_amorphous_crystal(seed=2)
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([512, 512]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    assert sim.num_slices == 150
+    sim.build_probe(sim_probe())
+    sim.generate_probe_positions(grid_steps=np.array([8, 4]))
+    sim.plan_simulation()
+    sim.multislice(bandwidth=2.0)
+    tot = sim.check_simulation()
+    np.testing.assert_allclose(tot, 1.0, rtol=0, atol=5e-5)
+
+
+def test_config2_full_size_parity_two_probes(gpu):
+    """Full config-2 workload (512^2, 150 slices, 73,728 atoms) for two raster
+    points against the FP64 oracle."""
+    _say("config2 full-size parity")
+    rec = structures.random_amorphous_crystal(seed=2)
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([512, 512]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    sim.build_probe(sim_probe())
+    sim.generate_probe_positions(grid_steps=np.array([64, 64]))
+    pos = sim.probe_positions[[0, 2080]]
+    sim.probe_positions = pos
+    sim.num_probes = np.int32(2)
+    sim.grid_steps = np.array([2, 1])
+    sim.plan_simulation()
+    sim.multislice()
+    t_ref, _ = oracle_slices(sim, 2.0)
+    psi_k, _ = oracle_probe(sim)
+    cb_ref = O.cbed(psi_k, pos.astype(np.float32), np.float32(sim.kmax), t_ref, oracle_pm(sim, 2.0, 1 / 3))
+    errs = [rel_l2(sim.probes[i], cb_ref[i]) for i in range(2)]
+    _say(f"  config2 CBED relL2 {errs}")
+    assert max(errs) < 1e-4
+
+
+def test_vacuum_closed_form_2048(gpu):
+    """No slices (S=0) and transparent slices both give mask-free / masked |psi_k|^2."""
+    import torch
+
+    _say("vacuum 2048^2")
+    N = 2048
+    rec = random_block(4, 128.0, 4.0, 50)
+    sim = MSAGPU(300, 0.03, rec, sampling=np.array([N, N]), debye_waller=False)
+    sim.setup_device(0)
+    sim.slice_t = 1.0
+    sim.build_probe(sim_probe())
+    psi_k, _ = oracle_probe(sim)
+    pos = torch.tensor([[0.0, 0.0], [10.0, -20.0]], dtype=torch.float32, device=gpu)
+    cb = torch.empty((2, N, N), dtype=torch.float32, device=gpu)
+    ws = torch.empty((2, N, N), dtype=torch.complex64, device=gpu)
+    st = L.stream_handle()
+    L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos), 2, N, N, np.float32(sim.kmax), None, 0,
+           None, L.OUT_CBED, np.float32(1 / sim.normalization), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
+    expect = np.abs(psi_k) ** 2 / np.sum(np.abs(psi_k) ** 2)
+    for i in range(2):
+        assert rel_l2(cb[i].cpu().numpy(), expect) < 1e-5
+    ones = torch.ones((3, N, N), dtype=torch.complex64, device=gpu)
+    pm = sim.build_propagator(1 / 3)
+    L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos), 2, N, N, np.float32(sim.kmax), L.ptr(ones), 3,
+           L.ptr(pm), L.OUT_CBED, np.float32(1 / sim.normalization), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
+    masked = expect * (np.abs(oracle_pm(sim, 1.0, 1 / 3)) > 0)
+    for i in range(2):
+        assert rel_l2(cb[i].cpu().numpy(), masked) < 1e-5
+
+
+def test_config1_si111_parity(gpu):
+    """Config-1 workload: Si[111] 256^2, ~104 slices of 1.92 A, one centred probe, 100 kV."""
+    _say("config1 Si[111]")
+    rec = structures.si111_supercell()
+    sim = MSAGPU(100, 0.020, rec, sampling=np.array([256, 256]), debye_waller=False)
+    _run_sim(sim, 1.92, np.array([[0.0, 0.0]]))
+    assert sim.num_slices == 104
+    t_ref, _ = oracle_slices(sim, 1.92)
+    psi_k, _ = oracle_probe(sim)
+    cb_ref = O.cbed(psi_k, np.zeros((1, 2), np.float32), np.float32(sim.kmax), t_ref, oracle_pm(sim, 1.92, 1 / 3))
+    err = rel_l2(sim.probes[0], cb_ref[0])
+    _say(f"  config1 CBED relL2 {err:.2e}")
+    assert err < 1e-4

@@ -1,0 +1,114 @@
+"""Host-side logic of the drop-in class against the oracle (CPU only, no GPU calls)."""
+import numpy as np
+import pytest
+
+from namsa_amd import MSAGPU, structures
+from namsa_amd.optics import sigma_int, voltage2Lambda
+from oracle import msa_oracle as O
+
+
+def _sim(rec, N=256, kv=200.0, semi=0.025):
+    return MSAGPU(kv, semi, rec, sampling=np.array([N, N]), debye_waller=False)
+
+
+@pytest.fixture(scope="module")
+def config2():
+    return structures.random_amorphous_crystal(seed=2)
+
+
+def test_optics_match_oracle():
+    for v in (60e3, 100e3, 200e3, 300e3):
+        assert voltage2Lambda(v) == O.voltage2lambda(v)
+        assert sigma_int(v) == pytest.approx(O.sigma_int(v), rel=1e-15)
+
+
+def test_geometry_matches_oracle(config2):
+    sim = _sim(config2, N=512)
+    g = O.geometry(sim.supercell_xyz, (512, 512), 200.0)
+    np.testing.assert_array_equal(sim.dims, g["dims"])
+    assert sim.kmax == g["kmax"] and sim.max_ang == g["max_ang"]
+    np.testing.assert_array_equal(sim.pix_size, g["pix_size"])
+    assert sim.sigma == pytest.approx(g["sigma"], rel=1e-15)
+
+
+def test_config2_structure_invariants(config2):
+    from scipy.spatial import cKDTree
+
+    xyz = np.stack([config2["x"], config2["y"], config2["z"]], 1).astype(np.float64)
+    assert len(config2) == 73728                       # 64*64*300*0.06
+    assert set(np.unique(config2["atomic_number"])) == {6, 14, 29, 79}
+    d, _ = cKDTree(xyz).query(xyz, k=2)
+    assert d[:, 1].min() >= 1.2 - 1e-4
+    assert xyz[:, 2].max() - xyz[:, 2].min() == pytest.approx(300.0, abs=1e-4)
+    assert int(np.floor(np.float64(np.float32(xyz[:, 2].max() - xyz[:, 2].min())) / 2.0)) == 150
+
+
+def test_si111_structure():
+    rec = structures.si111_supercell()
+    xyz = np.stack([rec["x"], rec["y"], rec["z"]], 1)
+    assert set(rec["atomic_number"]) == {14}
+    dims = xyz.max(0) - xyz.min(0)
+    assert 36 < dims[0] < 39 and 37 < dims[1] < 40 and 199 < dims[2] < 200
+    assert int(np.floor(np.float64(dims[2]) / 1.92)) == 104
+    from scipy.spatial import cKDTree
+
+    d, _ = cKDTree(xyz.astype(np.float64)).query(xyz.astype(np.float64), k=2)
+    assert d[:, 1].min() == pytest.approx(5.431 * np.sqrt(3) / 4, rel=1e-4)  # Si-Si bond
+
+
+def test_site_table_matches_oracle_binning(config2):
+    sim = _sim(config2, N=512)
+    sim.unique_Z = np.unique(sim.supercell_Z)
+    sites, offs = sim.site_table(2.0)
+    ref, uniq = O.bin_sites(sim.supercell_xyz, sim.supercell_Z, 2.0, sim.pix_size, sim.dims)
+    assert len(offs) - 1 == len(ref) == 150
+    for s in range(len(ref)):
+        np.testing.assert_array_equal(sites[offs[s]:offs[s + 1]], ref[s])
+
+
+def test_site_table_boundaries_and_negative_coordinates():
+    xyz = np.array([[-0.3, 1.0, 0.0], [2.0, -0.9, 1.0], [5.0, 5.0, 2.0], [9.9, 9.9, 4.0], [0.0, 0.0, 8.0],
+                    [10.0, 10.0, 3.999999]], np.float32)
+    rec = structures.make_records(xyz, [6, 14, 6, 79, 14, 6])
+    sim = MSAGPU(100.0, 0.02, rec, sampling=np.array([128, 128]), debye_waller=False)
+    sim.unique_Z = np.unique(sim.supercell_Z)
+    sites, offs = sim.site_table(2.0)
+    ref, _ = O.bin_sites(sim.supercell_xyz, sim.supercell_Z, 2.0, sim.pix_size, sim.dims)
+    for s in range(len(ref)):
+        np.testing.assert_array_equal(sites[offs[s]:offs[s + 1]], ref[s])
+
+
+def test_probe_positions_match_oracle(config2):
+    sim = _sim(config2, N=512)
+    sim.generate_probe_positions(grid_steps=np.array([64, 64]))
+    ref, gs = O.probe_positions(sim.dims, grid_steps=np.array([64, 64]))
+    np.testing.assert_array_equal(sim.probe_positions, ref)
+    assert sim.num_probes == 4096 and list(sim.grid_steps) == [64, 64]
+    sim.generate_probe_positions(probe_step=np.array([2.0, 4.0]))
+    ref2, gs2 = O.probe_positions(sim.dims, probe_step=np.array([2.0, 4.0]))
+    np.testing.assert_array_equal(sim.probe_positions, ref2)
+
+
+def test_detector_masks(config2):
+    sim = _sim(config2, N=512)
+    ref = O.detector_mask_reference((512, 512), sim.kmax, sim.Lambda, 40e-3, 200e-3)
+    np.testing.assert_array_equal(sim.detector_mask(40e-3, 200e-3), ref)
+    phys = sim.detector_mask(0.0, 25e-3, mode="physical")
+    theta = O.k_radius(512, 512, np.float32(sim.kmax)) * sim.Lambda
+    np.testing.assert_array_equal(phys, (theta < 25e-3).astype(np.float32))
+    with pytest.raises(ValueError):
+        sim.detector_mask(40e-3, 0.3)
+
+
+def test_table_shape_rule(config2):
+    sim = _sim(config2, N=512)
+    shape = tuple(int(np.floor(8 / p)) for p in sim.pix_size)
+    assert shape == O.table_shape(sim.pix_size) == (64, 64)
+
+
+def test_product_does_not_import_oracle():
+    import pathlib
+
+    for f in pathlib.Path("namsa_amd").rglob("*.py"):
+        src = f.read_text()
+        assert "import oracle" not in src and "from oracle" not in src, f

@@ -1,0 +1,307 @@
+"""Pins for the FP64 oracle: known-answer values, closed forms, invariants and
+brute force, chosen so that a dropped term, wrong sign, wrong index or
+transposed axis in the oracle fails at least one of them.  CPU only."""
+import numpy as np
+import pytest
+from scipy import integrate
+from scipy.special import k0
+
+from oracle import cpu_port
+from oracle import msa_oracle as O
+
+KIRK = np.load("namsa_amd/data/kirkland_params.npy")
+
+
+# ------------------------------------------------------------------ optics KATs
+@pytest.mark.parametrize("kv,lam,sigma", [(100, 0.0370145, 9.2439e-4), (200, 0.0250795, 7.2884e-4),
+                                          (300, 0.0196876, 6.5261e-4)])
+def test_wavelength_and_sigma_kats(kv, lam, sigma):
+    # textbook relativistic wavelengths / interaction parameters
+    assert O.voltage2lambda(kv * 1e3) == pytest.approx(lam, rel=2e-6)
+    assert O.sigma_int(kv * 1e3) == pytest.approx(sigma, rel=2e-4)
+
+
+def test_scherzer_defocus_closed_form():
+    assert O.scherzer_defocus(0.025, 1.0e7) == pytest.approx(np.sqrt(1.5 * 1.0e7 * 0.025))
+
+
+# ------------------------------------------------------------------ potentials
+def test_bessel_k0_known_value():
+    assert k0(1.0) == pytest.approx(0.42102443824070834, rel=1e-12)
+
+
+@pytest.mark.parametrize("Z", [6, 14, 29, 79])
+def test_projected_potential_plane_integral(Z):
+    # int v d^2r = 2 pi a0 e (sum a/b + sum c) when the Bessel argument has no
+    # offset: int K0(alpha r) 2 pi r dr = 2 pi / alpha^2, int exp(-pi^2 r^2/d) 2 pi r dr = d/pi
+    p = KIRK[Z - 1]
+    a, b, c, d = p
+    expect = 2 * np.pi * O.A0_BOHR * O.E_CHARGE * (np.sum(a / b) + np.sum(c))
+    f = lambda r: O.projected_potential(np.array([r]), p, eps=0.0)[0] * 2 * np.pi * r
+    val = sum(integrate.quad(f, lo, hi, limit=400)[0] for lo, hi in [(0, 0.01), (0.01, 1), (1, 10), (10, 60)])
+    assert val == pytest.approx(expect, rel=1e-6)
+
+
+def test_projected_potential_gaussian_term_at_origin():
+    # with all a = 0 the potential is 2 pi^2 a0 e sum(c/d) exp(-pi^2 r^2/d)
+    p = KIRK[13].copy()
+    p[0] = 0
+    r = np.array([0.0, 0.37])
+    v = O.projected_potential(r, p)
+    c, d = p[2], p[3]
+    pref = 2 * np.pi ** 2 * O.A0_BOHR * O.E_CHARGE
+    assert v[0] == pytest.approx(pref * np.sum(c / d), rel=1e-12)
+    assert v[1] == pytest.approx(pref * np.sum(c / d * np.exp(-np.pi ** 2 * 0.37 ** 2 / d)), rel=1e-12)
+
+
+def test_atomic_table_shape_min_symmetry_and_scale():
+    pix = np.array([0.125, 0.125])
+    t = O.atomic_table(KIRK[13], pix)
+    assert t.shape == (64, 64) and t.dtype == np.float32
+    assert t.min() == 0.0
+    np.testing.assert_allclose(t, t[::-1, :], rtol=0, atol=1e-3)
+    np.testing.assert_allclose(t, t.T, rtol=0, atol=1e-3)
+    # radially decreasing along the central row
+    row = t[32, 32:]
+    assert np.all(np.diff(row) <= 1e-3)
+    # plane integral over the 8 A box vs a direct quadrature of v - v_min
+    v_corner = O.projected_potential(np.array([np.hypot(4 - 8 / 256, 4 - 8 / 256)]), KIRK[13])[0]
+    box = integrate.dblquad(lambda y, x: O.projected_potential(np.array([np.hypot(x, y)]), KIRK[13])[0],
+                            -4, 4, -4, 4, epsabs=1e-3)[0]
+    est = (t.astype(np.float64).sum() + 0) * (8 / 64) ** 2
+    # binned table integral ~ box integral minus v_min * area (v_min ~ corner bin mean)
+    assert est == pytest.approx(box - v_corner * 64, rel=2e-2)
+
+
+def test_atomic_table_odd_shape_rule():
+    pix = np.array([0.15, 0.148])
+    t = O.atomic_table(KIRK[13], pix)
+    assert t.shape == (int(8 / 0.15), int(8 / 0.148)) == (53, 54)
+
+
+# ------------------------------------------------------------------ slicing
+def test_bin_sites_known_answers():
+    xyz = np.array([[1.0, 2.0, 0.0],     # slice 0
+                    [3.9, 0.05, 1.999],  # slice 0
+                    [0.3, 0.3, 2.0],     # slice 1 (lower bound inclusive)
+                    [7.99, 7.99, 5.5],   # slice 2
+                    [0.0, 0.0, 6.0]],    # beyond floor(6/2)=3 slices -> dropped
+                   dtype=np.float32)
+    Z = np.array([14, 6, 14, 79, 6])
+    sites, uniq = O.bin_sites(xyz, Z, 2.0, np.array([0.25, 0.5]), dims=np.array([8, 8, 6.0]))
+    assert list(uniq) == [6, 14, 79]
+    assert [len(s) for s in sites] == [2, 1, 1]
+    np.testing.assert_array_equal(sites[0], [[1, 8, 2], [0, 0, 7]])   # (zidx, int(y/py), int(x/px))
+    np.testing.assert_array_equal(sites[1], [[1, 1, 0]])
+    np.testing.assert_array_equal(sites[2], [[2, 31, 15]])
+
+
+def test_num_slices_notebook_kat():
+    # Si[111] notebook: dims z 486.47 A, dz 3.15 -> 154 slices (Si111_msa_cpu.ipynb)
+    assert O.num_slices(486.47, 3.15) == 154
+
+
+def _table(P0, P1, seed=0):
+    return np.random.default_rng(seed).random((2, P0, P1)).astype(np.float32) + 1
+
+
+def test_splat_single_atom_interior_even():
+    tab = _table(6, 4)
+    V = O.splat_slices(tab, [np.array([[1, 10, 20]])], 32, 40)
+    ref = np.zeros((32, 40))
+    ref[7:13, 18:22] = tab[1]             # y0 = 10-3, x0 = 20-2
+    np.testing.assert_array_equal(V[0], ref)
+
+
+def test_splat_odd_table_round_half_even_and_clipping():
+    tab = _table(53, 53)
+    # yc = 10: y0 = rint(-16.5) = -16, y1 = rint(36.5) = 36 -> rows [0,36) get table rows [16,52)
+    # xc = 11: x0 = rint(-15.5) = -16, x1 = rint(37.5) = 38 -> cols [0,37) (capped by P) get [16,53)
+    V = O.splat_slices(tab, [np.array([[0, 10, 11]])], 64, 64)
+    ref = np.zeros((64, 64))
+    ref[0:36, 0:37] = tab[0, 16:52, 16:53]
+    np.testing.assert_array_equal(V[0], ref)
+    # bottom-right edge clipping
+    V2 = O.splat_slices(tab, [np.array([[0, 60, 62]])], 64, 64)
+    y0, x0 = int(np.rint(np.float32(60 - 26.5))), int(np.rint(np.float32(62 - 26.5)))
+    ref2 = np.zeros((64, 64))
+    ref2[y0:64, x0:64] = tab[0, :64 - y0, :64 - x0]
+    np.testing.assert_array_equal(V2[0], ref2)
+
+
+def test_splat_linearity_and_sentinel_skip():
+    tab = _table(8, 8, 1)
+    a = np.array([[0, 5, 5]])
+    b = np.array([[1, 9, 3]])
+    both = O.splat_slices(tab, [np.concatenate([a, b, [[-1, -1, -1]]])], 16, 16)
+    np.testing.assert_allclose(both, O.splat_slices(tab, [a], 16, 16) + O.splat_slices(tab, [b], 16, 16))
+
+
+def test_transmission_unit_modulus_and_nan_guard():
+    V = np.array([[[0.0, 1.5e3, np.nan]]])
+    t = O.transmission(V, 7.2884e-4)
+    assert t[0, 0, 0] == 1
+    assert abs(abs(t[0, 0, 1]) - 1) < 1e-15
+    assert t[0, 0, 2] == 1 + 1j
+
+
+# ------------------------------------------------------------------ probe / propagator
+def test_k_axis_linspace_convention():
+    k = O.k_axis(8, 8.0)
+    np.testing.assert_allclose(k, np.linspace(-4, 4, 8), rtol=0, atol=1e-15)
+    assert k[0] == -4 and k[-1] == pytest.approx(4)
+
+
+def test_probe_amplitude_is_aperture_and_defocus_is_propagation():
+    N, km, lam = 64, 6.0, 0.0250795
+    psi_k, A = O.probe_k(N, N, km, 0.9, lam, C1=35.0)
+    np.testing.assert_allclose(np.abs(psi_k), A, rtol=1e-12)
+    # a pure defocus C1 is a Fresnel propagation: exp(-i chi) = conj(P(dz=C1)) up to
+    # the float32 pi of the aberration kernel and float32 k of the propagator
+    P = O.propagator_mask(N, N, km, 100.0, lam, 35.0)
+    np.testing.assert_allclose(psi_k / A, np.conj(P), rtol=0, atol=2e-5)
+
+
+def test_hard_aperture_and_mask_are_float32_comparisons():
+    N, km = 32, 7.0
+    k32 = O.k_radius(N, N, km).astype(np.float32)
+    m = O.bandwidth_mask(N, N, km, np.float32(km / 3))
+    np.testing.assert_array_equal(m, (k32 < np.float32(km / 3)).astype(float))
+    _, A = O.probe_k(N, N, km, 1.0, 0.02, smooth=False)
+    np.testing.assert_array_equal(A, (k32 < np.float32(1.0)).astype(float))
+
+
+def test_propagator_composition():
+    N, km, lam = 48, 8.0, 0.0196876
+    a = O.propagator_mask(N, N, km, km, lam, 0.75)
+    b = O.propagator_mask(N, N, km, km, lam, 1.25)
+    ab = O.propagator_mask(N, N, km, km, lam, 2.0)
+    np.testing.assert_allclose(a * b, ab, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.abs(a), O.bandwidth_mask(N, N, km, km), rtol=0, atol=1e-14)
+
+
+def test_dft_conventions_brute_force():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8))
+    n = np.arange(8)
+    W = np.exp(-2j * np.pi * np.outer(n, n) / 8)
+    np.testing.assert_allclose(O.fft2_u(x), W @ x @ W.T, atol=1e-12)
+    np.testing.assert_allclose(O.ifft2_u(x), np.conj(W) @ x @ np.conj(W).T, atol=1e-12)
+    # fftshift of an inverse transform == (-1)^(ky+kx) modulation of the spectrum
+    chk = (-1.0) ** (n[:, None] + n[None, :])
+    np.testing.assert_allclose(O.fftshift_even(O.ifft2_u(x)), O.ifft2_u(x * chk), atol=1e-12)
+
+
+def test_probe_parseval_and_stack_origin():
+    N, km = 32, 7.5
+    psi_k, _ = O.probe_k(N, N, km, 1.2, 0.025)
+    psi, norm = O.probe_real(psi_k)
+    assert norm == pytest.approx(N * N * np.sum(np.abs(psi_k) ** 2), rel=1e-12)
+    st = O.probe_stack(psi_k, np.zeros((1, 2), np.float32), km)
+    np.testing.assert_allclose(st[0], psi, atol=1e-12)
+
+
+# ------------------------------------------------------------------ multislice
+def _setup(N=32, km=7.0, S=3, seed=0, full_mask=False):
+    rng = np.random.default_rng(seed)
+    psi_k, _ = O.probe_k(N, N, km, 1.1, 0.0250795)
+    pos = rng.uniform(-2, 2, (3, 2)).astype(np.float32)
+    kcut = 10 * km if full_mask else km / 3
+    PM = O.propagator_mask(N, N, km, kcut, 0.0250795, 2.0)
+    t = np.exp(1j * rng.uniform(0, 2 * np.pi, (S, N, N)))
+    return psi_k, pos, PM, t
+
+
+def test_vacuum_cbed_closed_form():
+    psi_k, pos, PM, t = _setup()
+    cb = O.cbed(psi_k, pos, 7.0, np.ones_like(t), PM)
+    expect = np.abs(PM) * np.abs(psi_k) ** 2 / np.sum(np.abs(psi_k) ** 2)
+    for c in cb:
+        np.testing.assert_allclose(c, expect, rtol=0, atol=1e-14)
+
+
+def test_unitarity_with_full_mask():
+    psi_k, pos, PM, t = _setup(full_mask=True)
+    cb = O.cbed(psi_k, pos, 7.0, t, PM)
+    np.testing.assert_allclose(cb.sum(axis=(1, 2)), 1.0, rtol=1e-12)
+
+
+def test_tilt_slice_shifts_cbed_along_x():
+    N, km, m = 32, 7.0, 3
+    psi_k, _ = O.probe_k(N, N, km, 1.1, 0.0250795)
+    ones = O.propagator_mask(N, N, km, 10 * km, 0.0250795, 0.0)
+    x = np.arange(N)
+    t = np.exp(2j * np.pi * m * x / N)[None, None, :] * np.ones((1, N, N))
+    cb = O.cbed(psi_k, np.zeros((1, 2), np.float32), km, t, ones)[0]
+    expect = np.roll(np.abs(psi_k) ** 2, m, axis=1) / np.sum(np.abs(psi_k) ** 2)
+    np.testing.assert_allclose(cb, expect, atol=1e-14)
+
+
+def test_multislice_brute_force_n8():
+    rng = np.random.default_rng(5)
+    N = 8
+    n = np.arange(N)
+    W = np.exp(-2j * np.pi * np.outer(n, n) / N)
+    psi = rng.standard_normal((1, N, N)) + 1j * rng.standard_normal((1, N, N))
+    t = np.exp(1j * rng.uniform(0, 6, (2, N, N)))
+    PM = np.exp(1j * rng.uniform(0, 6, (N, N)))
+    ref = psi[0]
+    for i in range(2):
+        ref = np.conj(W) @ (PM * (W @ (ref * t[i] / N ** 2) @ W.T)) @ np.conj(W).T
+    np.testing.assert_allclose(O.multislice(psi, t, PM)[0], ref, atol=1e-10)
+    desc = psi[0]
+    for i in (1, 0):
+        desc = np.conj(W) @ (PM * (W @ (desc * t[i] / N ** 2) @ W.T)) @ np.conj(W).T
+    np.testing.assert_allclose(O.multislice(psi, t, PM, order="descending")[0], desc, atol=1e-10)
+
+
+# ------------------------------------------------------------------ raster / detectors
+def test_probe_positions_grid_kat():
+    pos, gs = O.probe_positions(np.array([10.0, 10.0, 1.0]), grid_steps=np.array([3, 3]))
+    assert list(gs) == [3, 3]
+    np.testing.assert_allclose(pos[:4], [[-2.25, -2.25], [0.25, -2.25], [2.75, -2.25], [-2.25, 0.25]])
+    pos2, _ = O.probe_positions(np.array([138.0, 138.0, 486.0]), grid_steps=np.array([17, 17]))
+    assert len(pos2) == 289  # Si111 notebook raster
+
+
+def test_probe_positions_step_branch():
+    pos, gs = O.probe_positions(np.array([10.0, 20.0, 1.0]), probe_step=np.array([2.5, 5.0]))
+    assert list(gs) == [4, 4]
+    assert pos[0].tolist() == [-10.0, -5.0] and pos[-1].tolist() == [10.0, 5.0]
+
+
+def test_detector_mask_reference_brute_force():
+    N, km, lam = 16, 6.0, 0.025
+    det = O.detector_mask_reference((N, N), km, lam, 0.02, 0.09)
+    max_ang = km * lam
+    ri, ro = N * 0.02 / max_ang / 2, N * 0.09 / max_ang / 2
+    for i in range(N):
+        for j in range(N):
+            r = np.hypot(i - N // 2, j - N // 2)
+            assert det[i, j] == float((r < ro) != (r < ri))
+
+
+def test_integrate_cbed_brute_force():
+    rng = np.random.default_rng(2)
+    cb = rng.random((6, 8, 8))
+    det = (rng.random((8, 8)) > 0.5).astype(float)
+    sig = O.integrate_cbed(cb, np.array([2, 3]), det)
+    assert sig.shape == (2, 3)
+    assert sig[1, 2] == pytest.approx(np.sum(cb[5] * det))
+
+
+# ------------------------------------------------------------------ CPU baseline port
+def test_cpu_port_matches_oracle_where_conventions_coincide():
+    N, km, lam, semi = 32, 7.0, 0.0250795, 0.025
+    rng = np.random.default_rng(9)
+    V = np.repeat(rng.uniform(0, 50, (1, N, N)), 2, axis=0).astype(np.float32)  # symmetric stack
+    sigma = 7.2884e-4
+    exit_cpu = cpu_port.multislice_cpu([(0.0, 0.0)], V, sigma, km, lam, 2.0, semi, bandwidth=2.0,
+                                       probe_kw=dict(apert_smooth=80.0, scherzer=False), processes=1)
+    cb_cpu = np.abs(np.fft.fft2(exit_cpu[0])) ** 2 / (N * N)
+    psi_k, _ = O.probe_k(N, N, km, np.float32(semi / lam), lam)
+    PM = O.propagator_mask(N, N, km, 10 * km, lam, 2.0)
+    t = O.transmission(V, sigma)
+    cb = O.cbed(psi_k, np.zeros((1, 2), np.float32), km, t, PM)[0]
+    rel = np.linalg.norm(cb_cpu - cb) / np.linalg.norm(cb)
+    assert rel < 2e-4
