@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 1 GiB device copy first (HBM counter calibration)")
     p.add_argument("--kernel-reps", type=int, default=10)
+    p.add_argument("--tune", action="store_true",
+                   help="autotune schedules and the probe group first (default: compiled defaults)")
     return p.parse_args()
 
 
@@ -111,13 +113,19 @@ def main():
 
     import namsa_amd._lib as L
 
+    copy_gbs = None
     if args.copy_calibration:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         a.fill_(1.0)
         b.copy_(a)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        b.copy_(a)
+        c1.record()
         torch.cuda.synchronize()
-        log("copy calibration: 1 GiB read + 1 GiB write")
+        copy_gbs = 2 * a.numel() * 4 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+        log(f"copy calibration: 1 GiB read + 1 GiB write, {copy_gbs:.0f} GB/s")
         del a, b
 
     log("setup: structure, tables, slices, probe, raster")
@@ -129,7 +137,6 @@ def main():
     plane = Ny * Nx
     st = L.stream_handle()
     pos = torch.from_numpy(np.ascontiguousarray(sim.probe_positions, dtype=np.float32)).to(dev)
-    psi = torch.empty((P, Ny, Nx), dtype=torch.complex64, device=dev)
     cbed = torch.empty((P, Ny, Nx), dtype=torch.float32, device=dev)
     det = torch.from_numpy(dets).to(dev)
     lib = L.load()
@@ -138,12 +145,29 @@ def main():
     scale = np.float32(1.0 / sim.normalization)
     kmax = np.float32(sim.kmax)
 
+    from namsa_amd import autotune
+    from namsa_amd.msa import cache_group
+
+    lo, hi = sim.band
+    group = cache_group(Ny, Nx)
+    if args.tune:
+        log("tuning slice-step schedules at the probe group, then the group")
+        tuned = autotune.tune(Nx, batch=min(P, group), reps=5, write_cache=False, band=(lo, hi))
+        log(f"tuned: {tuned['best']}  row {tuned['row']}  col {tuned['col']}")
+        tg = autotune.tune_group(Nx, P, slices=6, write_cache=False)
+        group = int(tg["best"])
+        log(f"group times (ms, 6 slices): {tg['times_ms']} -> {group}")
+    sched = {"row": autotune.schedules(Nx, 0)[autotune.selected(Nx, 0)],
+             "col": autotune.schedules(Nx, 1)[autotune.selected(Nx, 1)], "group": group,
+             "band": [lo, hi]}
+    ws = torch.empty((min(group, P), Ny, Nx), dtype=torch.complex64, device=dev)
+
     def step():
         L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos), P, Ny, Nx, kmax, L.ptr(sim._t), S,
-               L.ptr(sim.pm_device), L.OUT_CBED, scale, L.ptr(psi), L.ptr(cbed), L.ptr(det), 2,
-               L.ptr(det_ws), L.ptr(det_out), st)
+               L.ptr(sim.pm_device), lo, hi, group, L.OUT_CBED, scale, L.ptr(ws), L.ptr(cbed), L.ptr(det),
+               2, L.ptr(det_ws), L.ptr(det_out), st)
 
-    log(f"warmup {args.warmup} step(s): {P} probes x {S} slices")
+    log(f"warmup {args.warmup} step(s): {P} probes x {S} slices, groups of {group}")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -162,27 +186,38 @@ def main():
     log(f"wall {wall:.3f} s, events {gpu_ms / 1e3:.3f} s, {value:.4g} probe-slices/s")
     sane = np.isfinite(det_out.cpu().numpy()).all()
 
-    # live per-kernel timing of the two fused passes on the stream they run on
-    L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), P, Ny, Nx, kmax, L.ptr(psi), st)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * args.kernel_reps)]
-    for r in range(args.kernel_reps):
+    # Live per-kernel timing: one group replayed launch by launch with HIP
+    # events on the stream the kernels run on (the same launches the timed
+    # region issues inside namsa_multislice).
+    G = min(group, P)
+    L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), G, Ny, Nx, kmax, L.ptr(ws), st)
+    reps = min(S, max(2, args.kernel_reps))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * reps)]
+    for r in range(reps):
+        b_lo, b_hi = (0, Nx) if r == 0 else (lo, hi)
         ev[3 * r].record()
-        L.call("namsa_slice_rows", L.ptr(psi), P, Ny, Nx, L.ptr(sim._t[r % S]), st)
+        L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[r]), b_lo, b_hi, st)
         ev[3 * r + 1].record()
-        L.call("namsa_slice_cols", L.ptr(psi), P, Ny, Nx, L.ptr(sim.pm_device), st)
+        L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
         ev[3 * r + 2].record()
     torch.cuda.synchronize()
-    row_ms = float(np.mean([ev[3 * r].elapsed_time(ev[3 * r + 1]) for r in range(args.kernel_reps)]))
-    col_ms = float(np.mean([ev[3 * r + 1].elapsed_time(ev[3 * r + 2]) for r in range(args.kernel_reps)]))
-    row_bytes = 16 * plane * P + 8 * plane      # wave R+W, slice read once
-    col_bytes = 16 * plane * P                  # wave R+W (propagator excluded)
+    steady = range(1, reps)  # slice 0 reads the full band
+    row_ms = float(np.mean([ev[3 * r].elapsed_time(ev[3 * r + 1]) for r in steady]))
+    col_ms = float(np.mean([ev[3 * r + 1].elapsed_time(ev[3 * r + 2]) for r in steady]))
+    frac = (hi - lo) / Nx
+    strip = 16 if "c16" in sched["col"] else (8 if "c8" in sched["col"] else 4)
+    strips = (-(-hi // strip) - lo // strip) * strip / Nx
+    row_bytes = int((8 * frac + 8) * plane * G + 8 * plane)  # band read + full write + slice
+    col_bytes = int(16 * strips * plane * G)                 # band strips read + write
+    nominal = 32 * plane * G + 8 * plane                     # two full round trips per slice
     kernels = {"row_pass": (row_ms, row_bytes), "col_pass": (col_ms, col_bytes)}
     dom = max(kernels, key=lambda k: kernels[k][0])
     d_ms, d_bytes = kernels[dom]
     achieved = d_bytes / (d_ms * 1e-3) / 1e9
     step_gbs = (row_bytes + col_bytes) / ((row_ms + col_ms) * 1e-3) / 1e9
-    log(f"row pass {row_ms:.3f} ms ({row_bytes / row_ms / 1e6:.0f} GB/s), col pass {col_ms:.3f} ms "
-        f"({col_bytes / col_ms / 1e6:.0f} GB/s), slice step {step_gbs:.0f} GB/s")
+    log(f"group {G}: row pass {row_ms * 1e3:.1f} us ({row_bytes / row_ms / 1e6:.0f} GB/s), col pass "
+        f"{col_ms * 1e3:.1f} us ({col_bytes / col_ms / 1e6:.0f} GB/s), slice step {step_gbs:.0f} GB/s, "
+        f"nominal {nominal / ((row_ms + col_ms) * 1e-3) / 1e9:.0f} GB/s")
 
     traffic = None
     tf = ROOT / "profiles" / "hbm_traffic_config2_512.json"
@@ -212,11 +247,13 @@ def main():
                                "(73,728 atoms, Z in {6,14,29,79}), 512x512 complex64, dz=2 A "
                                "(150 slices), 200 kV, 25 mrad, 64x64 raster, CBED + BF/ADF",
                    "grid": [Ny, Nx], "slices": S, "probe_positions": P, "batch": P,
-                   "method": "plain"},
+                   "method": "plain", "schedules": sched},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": d_bytes, "avg_launch_ms": d_ms,
-                     "slice_step_GBs": step_gbs, "row_ms": row_ms, "col_ms": col_ms},
+                     "bytes_per_launch": d_bytes, "avg_launch_ms": d_ms, "launch_probes": G,
+                     "slice_step_GBs": step_gbs,
+                     "slice_step_nominal_GBs": nominal / ((row_ms + col_ms) * 1e-3) / 1e9,
+                     "row_ms": row_ms, "col_ms": col_ms, "copy_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "finite_outputs": bool(sane),
     }

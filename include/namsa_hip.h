@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define NAMSA_ABI_VERSION 1
+#define NAMSA_ABI_VERSION 2
 
 #define NAMSA_OK 0
 #define NAMSA_ERR_ARG (-1)
@@ -43,6 +43,23 @@ int namsa_init(int device);
 
 /* 1 if n is a compiled transform length, else 0. */
 int namsa_fft_supported(int n);
+
+/*
+ * Compiled schedules of the two passes of the slice step, per transform
+ * length (pass 0 = row pass, 1 = column pass).  No reference counterpart: the
+ * reference specialises its kernels by Jinja2 constants and JIT-compiles them
+ * per run (namsa/cuda_kernels.py:8-56); here every length has several
+ * precompiled schedules and the autotuner (namsa_amd/autotune.py) picks one.
+ * The selection is process-wide; change it between launches, not during them.
+ *   namsa_sched_count  number of schedules (0 for an unsupported length)
+ *   namsa_sched_name   name of schedule v, or NULL
+ *   namsa_sched_get    selected schedule, or a negative status
+ *   namsa_sched_set    select schedule v
+ */
+int namsa_sched_count(int n, int pass);
+const char* namsa_sched_name(int n, int pass, int v);
+int namsa_sched_get(int n, int pass);
+int namsa_sched_set(int n, int pass, int v);
 
 /*
  * Kirkland projected-potential tables, one Py x Px float32 table per element
@@ -125,17 +142,21 @@ size_t namsa_det_workspace_doubles(int B, int Ny, int Nx, int nDet);
  * schedule slices themselves (graph capture, per-pass timing).  The wave
  * state between them is internal: after namsa_probe_init and after every
  * namsa_slice_cols it holds C^-1 of the spectrum of the current wave; after
- * namsa_slice_rows it holds the row transform of t_i * psi_i / N^2.
+ * namsa_slice_rows it holds the row transform of t_i * psi_i.
  *   namsa_probe_init  probe stack (build_probes_stack + IFFT + fftshift_2d_stack,
  *                     namsa/probe_kernels.cu:239-304, msa.py:959-964)
  *   namsa_slice_rows  first half of one slice (msa.py:973-976): row pass
  *   namsa_slice_cols  second half of one slice (msa.py:976-981): column pass
  * A slice i is namsa_slice_rows(t + i*Ny*Nx) followed by namsa_slice_cols.
+ * band_lo, band_hi: the columns of pm that may be nonzero (see below); the
+ * row pass right after namsa_probe_init must use the full band [0, Nx).
  */
 int namsa_probe_init(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
                      void* psi_ws, void* stream);
-int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, void* stream);
-int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, void* stream);
+int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo,
+                     int band_hi, void* stream);
+int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
+                     void* stream);
 
 /*
  * Batched multislice, fused (replaces MSAGPU.__propagate_beams,
@@ -147,20 +168,27 @@ int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, void* 
  *   pos_yx:  [B][2] float32 probe positions (y, x) in Angstrom
  *   t:       [S][Ny][Nx] complex64 transmission slices, applied in ascending order
  *   pm:      [Ny][Nx] complex64 propagator * mask
+ *   band_lo, band_hi: every column of pm outside [band_lo, band_hi) is zero
+ *            (the bandwidth mask); those columns are never computed.  (0, 0)
+ *            means the full width.
+ *   group:   probes pushed through all slices together (0 = all B); a group
+ *            whose waves fit the 256 MB last-level cache keeps the slice
+ *            loop out of HBM
  *   mode:    NAMSA_OUT_CBED     cbed[B][Ny][Nx] = |sqrt(N^2) FFT(psi_S/N^2)|^2 * out_scale
  *                               (out_scale = 1/normalization), plus
  *                               det_out[B][nDet] = sum(cbed * det[d]) if nDet > 0
  *            NAMSA_OUT_INCIDENT psi_ws[B][Ny][Nx] = incident probes * out_scale
  *            NAMSA_OUT_EXIT     psi_ws[B][Ny][Nx] = real-space exit waves * out_scale
- *   psi_ws:  [B][Ny][Nx] complex64 wave state (scratch or output)
+ *   psi_ws:  complex64 wave state: min(B, group) planes in NAMSA_OUT_CBED mode
+ *            (every group reuses them), B planes (the output) otherwise
  *   det:     [nDet][Ny][Nx] float32 detector weights (may be NULL if nDet == 0)
  *   det_ws:  namsa_det_workspace_doubles(B, Ny, Nx, nDet) float64 scratch values
  *   det_out: [B][nDet] float64
  */
 int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
-                     const void* t, int S, const void* pm, int mode, float out_scale,
-                     void* psi_ws, float* cbed, const float* det, int nDet, double* det_ws,
-                     double* det_out, void* stream);
+                     const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                     int mode, float out_scale, void* psi_ws, float* cbed, const float* det,
+                     int nDet, double* det_ws, double* det_out, void* stream);
 
 #ifdef __cplusplus
 }

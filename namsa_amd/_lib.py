@@ -13,7 +13,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().with_name("libnamsa_hip.so")
 HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "namsa_hip.h"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OUT_CBED = 0
 OUT_INCIDENT = 1
@@ -40,11 +40,15 @@ SIGNATURES = {
     "namsa_norm2": (_i, [_vp, _ll, _vp, _vp]),
     "namsa_det_workspace_doubles": (_sz, [_i, _i, _i, _i]),
     "namsa_integrate": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp]),
-    "namsa_multislice": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _f, _vp, _vp, _vp,
-                              _i, _vp, _vp, _vp]),
+    "namsa_multislice": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _i, _i, _i, _f, _vp, _vp,
+                              _vp, _i, _vp, _vp, _vp]),
+    "namsa_sched_count": (_i, [_i, _i]),
+    "namsa_sched_name": (C.c_char_p, [_i, _i, _i]),
+    "namsa_sched_get": (_i, [_i, _i]),
+    "namsa_sched_set": (_i, [_i, _i, _i]),
     "namsa_probe_init": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _vp]),
-    "namsa_slice_rows": (_i, [_vp, _i, _i, _i, _vp, _vp]),
-    "namsa_slice_cols": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "namsa_slice_rows": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
+    "namsa_slice_cols": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
 }
 
 _lib = None

@@ -21,7 +21,8 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 OBJDIR = PKG / "_build"
 LIB = PKG / "libnamsa_hip.so"
-SOURCES = ["abi_common.hip", "multislice.hip", "slices.hip"]
+SOURCES = ["abi_common.hip", "multislice.hip", "slices.hip", "passes.hip"]
+PASS_SIZES = [128, 192, 256, 384, 512, 768, 1024, 2048]
 ARCH = os.environ.get("NAMSA_OFFLOAD_ARCH", "gfx950")
 
 
@@ -41,9 +42,11 @@ def common_flags() -> list[str]:
     ]
 
 
-def _compile(src: str) -> Path:
-    obj = OBJDIR / (Path(src).stem + ".o")
-    cmd = [hipcc(), *common_flags(), "-c", str(CSRC / src), "-o", str(obj)]
+def _compile(job) -> Path:
+    src, defines = job
+    tag = "".join("_" + d.split("=")[-1] for d in defines)
+    obj = OBJDIR / (Path(src).stem + tag + ".o")
+    cmd = [hipcc(), *common_flags(), *(f"-D{d}" for d in defines), "-c", str(CSRC / src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{res.stdout}\n{res.stderr}")
@@ -53,8 +56,10 @@ def _compile(src: str) -> Path:
 def build(verbose: bool = True) -> Path:
     """Compile all sources (always) and link the shared library."""
     OBJDIR.mkdir(exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    jobs = [(s, []) for s in SOURCES] + [("passes_n.hip", [f"NAMSA_PASS_N={n}"]) for n in PASS_SIZES]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)

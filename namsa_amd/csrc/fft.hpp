@@ -158,16 +158,30 @@ struct BlockSync {
 
 // LDS accessors.  RowLds: one transform per contiguous (padded) region.
 // ColLds: C interleaved transforms, element (i, c) at i*C + c (padded).
+// An exchange is: write, barrier, read, then either a second barrier (single
+// buffer, so the next write cannot overtake a slow reader) or a switch to the
+// other of two buffers (DOUBLE): a later write to this buffer comes after the
+// next exchange's barrier, which every reader of this one has passed.
 struct RowLds {
+  static constexpr bool DOUBLE = false;
   float2* base;
   __device__ __forceinline__ float2& at(int i) const { return base[i + (i >> 4)]; }
+  __device__ __forceinline__ void flip() {}
   static constexpr int padded(int n) { return n + n / 16; }
 };
-template <int C>
+template <int C, bool DB = false>
 struct ColLds {
+  static constexpr bool DOUBLE = DB;
   float2* base;
   int c;
+  int other;  // DOUBLE: offset of the other buffer (+size or -size)
   __device__ __forceinline__ float2& at(int i) const { return base[i * C + c + ((i * C) >> 5)]; }
+  __device__ __forceinline__ void flip() {
+    if constexpr (DB) {
+      base += other;
+      other = -other;
+    }
+  }
   static constexpr int padded(int n) { return n * C + (n * C) / 32; }
 };
 
@@ -212,7 +226,7 @@ struct Fft<N, E, Radices<Rs...>> {
 
   // ---------------- one stage
   template <bool INV, int R, int NS, int OFF, bool LAST, class Lds, class Sync>
-  static __device__ __forceinline__ void stage(float2 (&v)[E], const Lds& lds, int t,
+  static __device__ __forceinline__ void stage(float2 (&v)[E], Lds& lds, int t,
                                                const float2 (&tw)[NTW_ALLOC]) {
     constexpr int Q = E / R;
     static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -244,16 +258,19 @@ struct Fft<N, E, Radices<Rs...>> {
       Sync::sync();
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = lds.at(t + T * m);
-      Sync::sync();
+      if constexpr (Lds::DOUBLE)
+        lds.flip();
+      else
+        Sync::sync();
     }
   }
 
   template <bool INV, int NS, int OFF, class Lds, class Sync>
-  static __device__ __forceinline__ void stages(float2 (&)[E], const Lds&, int,
+  static __device__ __forceinline__ void stages(float2 (&)[E], Lds&, int,
                                                 const float2 (&)[NTW_ALLOC]) {}
 
   template <bool INV, int NS, int OFF, class Lds, class Sync, int R, int... Rest>
-  static __device__ __forceinline__ void stages(float2 (&v)[E], const Lds& lds, int t,
+  static __device__ __forceinline__ void stages(float2 (&v)[E], Lds& lds, int t,
                                                 const float2 (&tw)[NTW_ALLOC]) {
     stage<INV, R, NS, OFF, sizeof...(Rest) == 0, Lds, Sync>(v, lds, t, tw);
     constexpr int NEXT_OFF = OFF + (NS > 1 ? (E / R) * (R - 1) : 0);
@@ -262,7 +279,7 @@ struct Fft<N, E, Radices<Rs...>> {
 
   // Full transform, natural layout in and out.
   template <bool INV, class Sync, class Lds>
-  static __device__ __forceinline__ void run(float2 (&v)[E], const Lds& lds, int t,
+  static __device__ __forceinline__ void run(float2 (&v)[E], Lds& lds, int t,
                                              const float2 (&tw)[NTW_ALLOC]) {
     stages<INV, 1, 0, Lds, Sync, Rs...>(v, lds, t, tw);
   }

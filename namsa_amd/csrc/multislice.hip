@@ -17,224 +17,12 @@
 // (msa.py:985-990) equals N^2 |PM * C(u_last)|^2 because FFT IFFT_u = N^2,
 // so the last column pass writes intensities instead of transforming back.
 #include <algorithm>
-#include <type_traits>
 
 #include "common.hpp"
-#include "fft.hpp"
 #include "namsa_hip.h"
+#include "passes.hpp"
 
 namespace namsa {
-
-// ------------------------------------------------------------------ schedules
-// Row transforms: T = N/E threads per row; T <= 64 keeps a row inside one
-// wavefront so its LDS exchanges need no workgroup barrier.
-template <int N>
-struct RowCfg;
-template <> struct RowCfg<128>  { static constexpr int E = 8;  using RL = Radices<8, 4, 4>; };
-template <> struct RowCfg<192>  { static constexpr int E = 12; using RL = Radices<3, 4, 4, 4>; };
-template <> struct RowCfg<256>  { static constexpr int E = 8;  using RL = Radices<8, 8, 4>; };
-template <> struct RowCfg<384>  { static constexpr int E = 12; using RL = Radices<3, 2, 4, 4, 4>; };
-template <> struct RowCfg<512>  { static constexpr int E = 8;  using RL = Radices<8, 8, 8>; };
-template <> struct RowCfg<768>  { static constexpr int E = 12; using RL = Radices<3, 4, 4, 4, 4>; };
-template <> struct RowCfg<1024> { static constexpr int E = 16; using RL = Radices<16, 8, 8>; };
-template <> struct RowCfg<2048> { static constexpr int E = 16; using RL = Radices<16, 16, 8>; };
-
-// Column transforms: C adjacent columns per workgroup so that each global
-// access of a wavefront covers C*8 contiguous bytes of C*8/64 rows.
-template <int N>
-struct ColCfg;
-template <> struct ColCfg<128>  { static constexpr int E = 8;  static constexpr int C = 16; using RL = Radices<8, 4, 4>; };
-template <> struct ColCfg<192>  { static constexpr int E = 12; static constexpr int C = 16; using RL = Radices<3, 4, 4, 4>; };
-template <> struct ColCfg<256>  { static constexpr int E = 8;  static constexpr int C = 16; using RL = Radices<8, 8, 4>; };
-template <> struct ColCfg<384>  { static constexpr int E = 12; static constexpr int C = 16; using RL = Radices<3, 2, 4, 4, 4>; };
-template <> struct ColCfg<512>  { static constexpr int E = 16; static constexpr int C = 16; using RL = Radices<8, 8, 8>; };
-template <> struct ColCfg<768>  { static constexpr int E = 12; static constexpr int C = 8;  using RL = Radices<3, 4, 4, 4, 4>; };
-template <> struct ColCfg<1024> { static constexpr int E = 16; static constexpr int C = 8;  using RL = Radices<16, 8, 8>; };
-template <> struct ColCfg<2048> { static constexpr int E = 16; static constexpr int C = 8;  using RL = Radices<16, 16, 8>; };
-
-constexpr int kRowThreads = 256;
-
-template <int N>
-constexpr int col_threads() { return ColCfg<N>::C * (N / ColCfg<N>::E); }
-
-enum RowOp { ROW_STEP = 0, ROW_INV = 1, ROW_FWD = 2 };
-enum ColOp { COL_PROP = 0, COL_PROBE = 1, COL_READOUT = 2, COL_FWD = 3, COL_INV = 4 };
-
-// ------------------------------------------------------------------ row pass
-// psi: [n_rows][N] (rows of all images back to back); tsl: [rows_per_image][N].
-template <int N, int OP>
-__global__ void __launch_bounds__(kRowThreads)
-row_kernel(float2* __restrict__ psi, long long n_rows, int rows_per_image,
-           const float2* __restrict__ tsl, float scale) {
-  constexpr int E = RowCfg<N>::E;
-  using F = Fft<N, E, typename RowCfg<N>::RL>;
-  constexpr int T = F::T;
-  constexpr int G = kRowThreads / T;
-  using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
-  __shared__ float2 smem[G * RowLds::padded(N)];
-
-  const int g = threadIdx.x / T;
-  const int t = threadIdx.x % T;
-  const RowLds lds{smem + g * RowLds::padded(N)};
-  float2 tw[F::NTW_ALLOC];
-  F::twiddles(t, tw);
-
-  const long long n_blocks = (n_rows + G - 1) / G;
-  for (long long rb = blockIdx.x; rb < n_blocks; rb += gridDim.x) {
-    const long long row = rb * G + g;
-    const bool active = row < n_rows;
-    const long long rr = active ? row : n_rows - 1;
-    float2* p = psi + rr * N;
-    float2 v[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = p[t + T * m];
-    if constexpr (OP == ROW_STEP) {
-      const float2* tr = tsl + static_cast<long long>(rr % rows_per_image) * N;
-      float2 tv[E];
-#pragma unroll
-      for (int m = 0; m < E; ++m) tv[m] = tr[t + T * m];
-      F::template run<true, Sync>(v, lds, t, tw);
-#pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cscale(cmul(v[m], tv[m]), scale);
-      F::template run<false, Sync>(v, lds, t, tw);
-    } else if constexpr (OP == ROW_INV) {
-      F::template run<true, Sync>(v, lds, t, tw);
-#pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cscale(v[m], scale);
-    } else {
-      F::template run<false, Sync>(v, lds, t, tw);
-    }
-    if (active) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) p[t + T * m] = v[m];
-    }
-  }
-}
-
-// ------------------------------------------------------------------ column pass
-struct ColArgs {
-  float2* psi;          // [B][Ny][Nx]
-  int Nx;
-  int tiles_per_image;  // Nx / C
-  long long n_tiles;    // B * tiles_per_image
-  const float2* pm;     // [Ny][Nx] propagator * mask (PROP, READOUT)
-  const float2* psi_k;  // [Ny][Nx] (PROBE)
-  const float* pos;     // [B][2] (y, x) (PROBE)
-  double kmax;          // (PROBE)
-  float amp;            // intensity factor (READOUT)
-  int apply_pm;         // (READOUT)
-  float* cbed;          // [B][Ny][Nx] (READOUT)
-  const float* det;     // [n_det][Ny][Nx] (READOUT)
-  int n_det;
-  double* det_ws;       // [n_tiles][n_det] (READOUT)
-};
-
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-  return x;
-}
-
-// exp(2 pi i * turns) with the phase reduced to [-1/2, 1/2] turns in double.
-__device__ __forceinline__ float2 unit_phase(double turns) {
-  const double f = turns - rint(turns);
-  double s, c;
-  sincospi(2.0 * f, &s, &c);
-  return make_float2(float(c), float(s));
-}
-
-template <int N, int OP>
-__global__ void __launch_bounds__(col_threads<N>())
-col_kernel(ColArgs a) {
-  constexpr int E = ColCfg<N>::E;
-  constexpr int C = ColCfg<N>::C;
-  using F = Fft<N, E, typename ColCfg<N>::RL>;
-  constexpr int T = F::T;
-  constexpr int NT = C * T;
-  constexpr int NW = NT / 64;
-  __shared__ float2 smem[ColLds<C>::padded(N)];
-  __shared__ double red[NW];
-
-  const int c = threadIdx.x % C;
-  const int jt = threadIdx.x / C;
-  const ColLds<C> lds{smem, c};
-  float2 tw[F::NTW_ALLOC];
-  F::twiddles(jt, tw);
-  const long long plane = static_cast<long long>(N) * a.Nx;
-
-  for (long long tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
-    const long long b = tile / a.tiles_per_image;
-    const int col = static_cast<int>(tile % a.tiles_per_image) * C + c;
-    float2* base = a.psi + b * plane + col;
-    float2 v[E];
-
-    if constexpr (OP == COL_PROBE) {
-      // psi_k * exp(2 pi i (-kx rx - ky ry)) * (-1)^(row + col)
-      // (build_probes_stack + fftshift_2d_stack, probe_kernels.cu:239-304)
-      const double ry = a.pos[2 * b];
-      const double rx = a.pos[2 * b + 1];
-      const float2 ex = unit_phase(-k_lin(col, a.kmax, a.Nx) * rx);
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int row = jt + T * m;
-        float2 e = cmul(ex, unit_phase(-k_lin(row, a.kmax, N) * ry));
-        if ((row + col) & 1) e = make_float2(-e.x, -e.y);
-        v[m] = cmul(a.psi_k[static_cast<long long>(row) * a.Nx + col], e);
-      }
-      F::template run<true, BlockSync>(v, lds, jt, tw);
-#pragma unroll
-      for (int m = 0; m < E; ++m) base[static_cast<long long>(jt + T * m) * a.Nx] = v[m];
-    } else {
-#pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = base[static_cast<long long>(jt + T * m) * a.Nx];
-
-      if constexpr (OP == COL_PROP) {
-        F::template run<false, BlockSync>(v, lds, jt, tw);
-#pragma unroll
-        for (int m = 0; m < E; ++m)
-          v[m] = cmul(v[m], a.pm[static_cast<long long>(jt + T * m) * a.Nx + col]);
-        F::template run<true, BlockSync>(v, lds, jt, tw);
-      } else if constexpr (OP == COL_FWD) {
-        F::template run<false, BlockSync>(v, lds, jt, tw);
-      } else if constexpr (OP == COL_INV) {
-        F::template run<true, BlockSync>(v, lds, jt, tw);
-      }
-
-      if constexpr (OP == COL_READOUT) {
-        F::template run<false, BlockSync>(v, lds, jt, tw);
-        float inten[E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-          const long long off = static_cast<long long>(jt + T * m) * a.Nx + col;
-          float2 y = v[m];
-          if (a.apply_pm) y = cmul(y, a.pm[off]);
-          inten[m] = (y.x * y.x + y.y * y.y) * a.amp;
-          a.cbed[b * plane + off] = inten[m];
-        }
-        for (int d = 0; d < a.n_det; ++d) {
-          const float* w = a.det + d * plane;
-          double s = 0.0;
-#pragma unroll
-          for (int m = 0; m < E; ++m)
-            s += double(inten[m]) * double(w[static_cast<long long>(jt + T * m) * a.Nx + col]);
-          s = wave_sum(s);
-          if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-          __syncthreads();
-          if (threadIdx.x == 0) {
-            double tot = 0.0;
-#pragma unroll
-            for (int i = 0; i < NW; ++i) tot += red[i];
-            a.det_ws[tile * a.n_det + d] = tot;
-          }
-          __syncthreads();
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < E; ++m) base[static_cast<long long>(jt + T * m) * a.Nx] = v[m];
-      }
-    }
-  }
-}
 
 // det_out[b][d] = sum over the column tiles of image b, in tile order.
 __global__ void det_finalize_kernel(const double* __restrict__ ws, int tiles_per_image, int n_det,
@@ -326,102 +114,34 @@ __global__ void norm2_kernel(const float2* __restrict__ x, long long n, double* 
   }
 }
 
-// ------------------------------------------------------------------ launchers
-template <class K>
-int persistent_grid(K kernel, int threads, long long work) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
-      per_cu <= 0)
-    per_cu = 1;
-  const long long cap = static_cast<long long>(per_cu) * device_cu_count();
-  return static_cast<int>(std::max<long long>(1, std::min(work, cap)));
-}
-
-template <int N, int OP>
-int launch_row(float2* psi, long long n_rows, int rows_per_image, const float2* tsl, float scale,
-               hipStream_t st) {
-  auto k = row_kernel<N, OP>;
-  constexpr int G = kRowThreads / (N / RowCfg<N>::E);
-  static const int cap = persistent_grid(k, kRowThreads, 1LL << 40);
-  const long long blocks = (n_rows + G - 1) / G;
-  const int grid = static_cast<int>(std::min<long long>(blocks, cap));
-  hipLaunchKernelGGL(k, dim3(grid), dim3(kRowThreads), 0, st, psi, n_rows, rows_per_image, tsl,
-                     scale);
-  return check_hip(hipGetLastError(), "row_kernel launch");
-}
-
-template <int N, int OP>
-int launch_col(const ColArgs& a, hipStream_t st) {
-  auto k = col_kernel<N, OP>;
-  static const int cap = persistent_grid(k, col_threads<N>(), 1LL << 40);
-  const int grid = static_cast<int>(std::min<long long>(a.n_tiles, cap));
-  hipLaunchKernelGGL(k, dim3(grid), dim3(col_threads<N>()), 0, st, a);
-  return check_hip(hipGetLastError(), "col_kernel launch");
-}
-
-#define NAMSA_FOR_EACH_N(X) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048)
-
-bool supported(int n) {
-  switch (n) {
-#define X(n) case n:
-    NAMSA_FOR_EACH_N(X)
-#undef X
-    return true;
-    default:
-      return false;
-  }
-}
-
-int row_pass(int Nx, int op, float2* psi, long long n_rows, int rows_per_image,
-             const float2* tsl, float scale, hipStream_t st) {
-  switch (Nx) {
-#define X(n)                                                                               \
-  case n:                                                                                  \
-    if (op == ROW_STEP) return launch_row<n, ROW_STEP>(psi, n_rows, rows_per_image, tsl, scale, st); \
-    if (op == ROW_INV) return launch_row<n, ROW_INV>(psi, n_rows, rows_per_image, tsl, scale, st);   \
-    return launch_row<n, ROW_FWD>(psi, n_rows, rows_per_image, tsl, scale, st);
-    NAMSA_FOR_EACH_N(X)
-#undef X
-    default:
-      return fail(kErrUnsupported, "unsupported row length %d", Nx);
-  }
-}
-
-int col_pass(int Ny, int op, const ColArgs& a, hipStream_t st) {
-  switch (Ny) {
-#define X(n)                                                       \
-  case n:                                                          \
-    switch (op) {                                                  \
-      case COL_PROP: return launch_col<n, COL_PROP>(a, st);        \
-      case COL_PROBE: return launch_col<n, COL_PROBE>(a, st);      \
-      case COL_READOUT: return launch_col<n, COL_READOUT>(a, st);  \
-      case COL_FWD: return launch_col<n, COL_FWD>(a, st);          \
-      default: return launch_col<n, COL_INV>(a, st);               \
-    }
-    NAMSA_FOR_EACH_N(X)
-#undef X
-    default:
-      return fail(kErrUnsupported, "unsupported column length %d", Ny);
-  }
-}
-
-int col_tile(int Ny) {
-  switch (Ny) {
-#define X(n) case n: return ColCfg<n>::C;
-    NAMSA_FOR_EACH_N(X)
-#undef X
-    default:
-      return 0;
-  }
-}
-
-ColArgs col_args(float2* psi, int B, int Ny, int Nx) {
+ColArgs col_args(float2* psi, int B, int Nx) {
   ColArgs a{};
   a.psi = psi;
+  a.B = B;
   a.Nx = Nx;
-  a.tiles_per_image = Nx / col_tile(Ny);
-  a.n_tiles = static_cast<long long>(B) * a.tiles_per_image;
+  a.band_lo = 0;
+  a.band_hi = Nx;
+  a.pm_scale = 1.0f;
   return a;
+}
+
+RowArgs row_args(float2* psi, int B, int Ny, int Nx, const float2* tsl, float scale, int lo, int hi) {
+  RowArgs r{};
+  r.psi = psi;
+  r.n_rows = static_cast<long long>(B) * Ny;
+  r.rows_per_image = Ny;
+  r.tsl = tsl;
+  r.scale = scale;
+  r.band_lo = lo;
+  r.band_hi = hi;
+  return r;
+}
+
+int check_band(int Nx, int& lo, int& hi, const char* who) {
+  if (lo == 0 && hi == 0) hi = Nx;
+  if (lo < 0 || hi > Nx || lo >= hi)
+    return fail(kErrArg, "%s: band [%d, %d) outside [0, %d)", who, lo, hi, Nx);
+  return kOk;
 }
 
 int check_grid(int Ny, int Nx, const char* who) {
@@ -436,21 +156,19 @@ using namespace namsa;
 
 extern "C" {
 
-int namsa_fft_supported(int n) { return supported(n) ? 1 : 0; }
-
 int namsa_fft2d(void* data, int B, int Ny, int Nx, int inverse, void* stream) {
   if (int rc = check_grid(Ny, Nx, "namsa_fft2d")) return rc;
   if (!data || B <= 0) return fail(kErrArg, "namsa_fft2d: null data or B=%d", B);
   hipStream_t st = as_stream(stream);
   float2* p = static_cast<float2*>(data);
-  ColArgs a = col_args(p, B, Ny, Nx);
-  const long long rows = static_cast<long long>(B) * Ny;
+  ColArgs a = col_args(p, B, Nx);
+  const RowArgs r = row_args(p, B, Ny, Nx, nullptr, 1.0f, 0, Nx);
   if (!inverse) {
-    if (int rc = row_pass(Nx, ROW_FWD, p, rows, Ny, nullptr, 1.0f, st)) return rc;
+    if (int rc = row_pass(Nx, ROW_FWD, r, st)) return rc;
     return col_pass(Ny, COL_FWD, a, st);
   }
   if (int rc = col_pass(Ny, COL_INV, a, st)) return rc;
-  return row_pass(Nx, ROW_INV, p, rows, Ny, nullptr, 1.0f, st);
+  return row_pass(Nx, ROW_INV, r, st);
 }
 
 int namsa_probe_k(int Ny, int Nx, float kmax, float k_semi, float lambda, float C1, float C3,
@@ -490,45 +208,53 @@ int namsa_integrate(const float* cbed, int B, int Ny, int Nx, const float* det, 
 }
 
 size_t namsa_det_workspace_doubles(int B, int Ny, int Nx, int nDet) {
-  const int c = col_tile(Ny);
-  if (c == 0 || B <= 0 || nDet <= 0) return 0;
-  return static_cast<size_t>(B) * (Nx / c) * nDet;
+  if (!supported(Ny) || !supported(Nx) || B <= 0 || nDet <= 0) return 0;
+  return static_cast<size_t>(B) * (Nx / kMinColTile) * nDet;
 }
 
 int namsa_probe_init(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
                      void* psi_ws, void* stream) {
   if (int rc = check_grid(Ny, Nx, "namsa_probe_init")) return rc;
   if (B <= 0 || !psi_k || !pos_yx || !psi_ws) return fail(kErrArg, "namsa_probe_init: bad arguments");
-  ColArgs a = col_args(static_cast<float2*>(psi_ws), B, Ny, Nx);
+  ColArgs a = col_args(static_cast<float2*>(psi_ws), B, Nx);
   a.psi_k = static_cast<const float2*>(psi_k);
   a.pos = pos_yx;
   a.kmax = double(kmax);
   return col_pass(Ny, COL_PROBE, a, as_stream(stream));
 }
 
-int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, void* stream) {
+int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo,
+                     int band_hi, void* stream) {
   if (int rc = check_grid(Ny, Nx, "namsa_slice_rows")) return rc;
   if (B <= 0 || !psi_ws || !t_slice) return fail(kErrArg, "namsa_slice_rows: bad arguments");
-  const float inv_n2 = 1.0f / static_cast<float>(static_cast<long long>(Ny) * Nx);
-  return row_pass(Nx, ROW_STEP, static_cast<float2*>(psi_ws), static_cast<long long>(B) * Ny, Ny,
-                  static_cast<const float2*>(t_slice), inv_n2, as_stream(stream));
+  if (int rc = check_band(Nx, band_lo, band_hi, "namsa_slice_rows")) return rc;
+  return row_pass(Nx, ROW_STEP,
+                  row_args(static_cast<float2*>(psi_ws), B, Ny, Nx,
+                           static_cast<const float2*>(t_slice), 1.0f, band_lo, band_hi),
+                  as_stream(stream));
 }
 
-int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, void* stream) {
+int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
+                     void* stream) {
   if (int rc = check_grid(Ny, Nx, "namsa_slice_cols")) return rc;
   if (B <= 0 || !psi_ws || !pm) return fail(kErrArg, "namsa_slice_cols: bad arguments");
-  ColArgs a = col_args(static_cast<float2*>(psi_ws), B, Ny, Nx);
+  if (int rc = check_band(Nx, band_lo, band_hi, "namsa_slice_cols")) return rc;
+  ColArgs a = col_args(static_cast<float2*>(psi_ws), B, Nx);
   a.pm = static_cast<const float2*>(pm);
+  a.pm_scale = 1.0f / static_cast<float>(static_cast<long long>(Ny) * Nx);
+  a.band_lo = band_lo;
+  a.band_hi = band_hi;
   return col_pass(Ny, COL_PROP, a, as_stream(stream));
 }
 
 int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
-                     const void* t, int S, const void* pm, int mode, float out_scale,
-                     void* psi_ws, float* cbed, const float* det, int nDet, double* det_ws,
-                     double* det_out, void* stream) {
+                     const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                     int mode, float out_scale, void* psi_ws, float* cbed, const float* det,
+                     int nDet, double* det_ws, double* det_out, void* stream) {
   if (int rc = check_grid(Ny, Nx, "namsa_multislice")) return rc;
-  if (B <= 0 || S < 0 || !psi_k || !pos_yx || !psi_ws)
-    return fail(kErrArg, "namsa_multislice: B=%d S=%d or null probe/positions/workspace", B, S);
+  if (B <= 0 || S < 0 || group < 0 || !psi_k || !pos_yx || !psi_ws)
+    return fail(kErrArg, "namsa_multislice: B=%d S=%d group=%d or null probe/positions/workspace",
+                B, S, group);
   if (mode != NAMSA_OUT_CBED && mode != NAMSA_OUT_INCIDENT && mode != NAMSA_OUT_EXIT)
     return fail(kErrArg, "namsa_multislice: unknown mode %d", mode);
   if (mode != NAMSA_OUT_INCIDENT && S > 0 && (!t || !pm))
@@ -536,46 +262,68 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
   if (mode == NAMSA_OUT_CBED && !cbed) return fail(kErrArg, "namsa_multislice: null cbed");
   if (mode == NAMSA_OUT_CBED && nDet > 0 && (!det || !det_ws || !det_out))
     return fail(kErrArg, "namsa_multislice: null detector buffers");
+  if (int rc = check_band(Nx, band_lo, band_hi, "namsa_multislice")) return rc;
 
   hipStream_t st = as_stream(stream);
-  float2* psi = static_cast<float2*>(psi_ws);
   const float2* tt = static_cast<const float2*>(t);
   const long long plane = static_cast<long long>(Ny) * Nx;
-  const long long rows = static_cast<long long>(B) * Ny;
   const float inv_n2 = 1.0f / static_cast<float>(plane);
+  const int G = (group > 0 && group < B) ? group : B;
 
-  ColArgs a = col_args(psi, B, Ny, Nx);
-  a.psi_k = static_cast<const float2*>(psi_k);
-  a.pos = pos_yx;
-  a.kmax = double(kmax);
-  a.pm = static_cast<const float2*>(pm);
-  if (int rc = col_pass(Ny, COL_PROBE, a, st)) return rc;
+  for (int g0 = 0; g0 < B; g0 += G) {
+    const int Bg = std::min(G, B - g0);
+    // CBED mode: every group reuses the first G planes of the workspace;
+    // otherwise the workspace is the output and each group owns its planes.
+    float2* psi = static_cast<float2*>(psi_ws) + (mode == NAMSA_OUT_CBED ? 0 : g0 * plane);
 
-  if (mode == NAMSA_OUT_INCIDENT) return row_pass(Nx, ROW_INV, psi, rows, Ny, nullptr, out_scale, st);
+    ColArgs a = col_args(psi, Bg, Nx);
+    a.psi_k = static_cast<const float2*>(psi_k);
+    a.pos = pos_yx + 2LL * g0;
+    a.kmax = double(kmax);
+    a.pm = static_cast<const float2*>(pm);
+    a.pm_scale = inv_n2;
+    a.band_lo = band_lo;
+    a.band_hi = band_hi;
+    if (int rc = col_pass(Ny, COL_PROBE, a, st)) return rc;
 
-  const int last_prop = (mode == NAMSA_OUT_EXIT) ? S : S - 1;
-  for (int i = 0; i < S; ++i) {
-    if (int rc = row_pass(Nx, ROW_STEP, psi, rows, Ny, tt + i * plane, inv_n2, st)) return rc;
-    if (i < last_prop)
-      if (int rc = col_pass(Ny, COL_PROP, a, st)) return rc;
-  }
-  if (mode == NAMSA_OUT_EXIT) return row_pass(Nx, ROW_INV, psi, rows, Ny, nullptr, out_scale, st);
-
-  // CBED readout.  S > 0: N^2 |PM C(u)|^2 / norm.  S == 0: the state is v0 and
-  // FFT(psi0) = Nx C(v0), so |sqrt(N^2) FFT(psi0/N^2)|^2 = (Nx/Ny) |C(v0)|^2.
-  a.apply_pm = S > 0 ? 1 : 0;
-  a.amp = S > 0 ? static_cast<float>(static_cast<double>(plane) * out_scale)
-                : static_cast<float>(static_cast<double>(Nx) / Ny * out_scale);
-  a.cbed = cbed;
-  a.det = det;
-  a.n_det = nDet > 0 ? nDet : 0;
-  a.det_ws = det_ws;
-  if (int rc = col_pass(Ny, COL_READOUT, a, st)) return rc;
-  if (nDet > 0) {
-    const int n = B * nDet;
-    hipLaunchKernelGGL(det_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, st, det_ws,
-                       a.tiles_per_image, nDet, B, det_out);
-    return check_hip(hipGetLastError(), "det_finalize_kernel launch");
+    if (mode == NAMSA_OUT_INCIDENT) {
+      if (int rc = row_pass(Nx, ROW_INV, row_args(psi, Bg, Ny, Nx, nullptr, out_scale, 0, Nx), st))
+        return rc;
+      continue;
+    }
+    // The probe former's spectrum fills every column; after the first
+    // propagator only the band can be nonzero.
+    const int last_prop = (mode == NAMSA_OUT_EXIT) ? S : S - 1;
+    for (int i = 0; i < S; ++i) {
+      const int lo = i == 0 ? 0 : band_lo, hi = i == 0 ? Nx : band_hi;
+      if (int rc = row_pass(Nx, ROW_STEP, row_args(psi, Bg, Ny, Nx, tt + i * plane, 1.0f, lo, hi), st))
+        return rc;
+      if (i < last_prop)
+        if (int rc = col_pass(Ny, COL_PROP, a, st)) return rc;
+    }
+    if (mode == NAMSA_OUT_EXIT) {
+      const int lo = S == 0 ? 0 : band_lo, hi = S == 0 ? Nx : band_hi;
+      if (int rc = row_pass(Nx, ROW_INV, row_args(psi, Bg, Ny, Nx, nullptr, out_scale, lo, hi), st))
+        return rc;
+      continue;
+    }
+    // CBED readout.  S > 0: the state is u = R(t x) (no 1/N^2 on the row
+    // pass), so |sqrt(N^2) FFT(psi_S/N^2)|^2 = |PM C(u)|^2 / N^2.  S == 0: the
+    // state is v0 and FFT(psi0) = Nx C(v0), so it is (Nx/Ny) |C(v0)|^2.
+    a.apply_pm = S > 0 ? 1 : 0;
+    a.amp = S > 0 ? static_cast<float>(out_scale / static_cast<double>(plane))
+                  : static_cast<float>(static_cast<double>(Nx) / Ny * out_scale);
+    a.cbed = cbed + g0 * plane;
+    a.det = det;
+    a.n_det = nDet > 0 ? nDet : 0;
+    a.det_ws = det_ws;
+    if (int rc = col_pass(Ny, COL_READOUT, a, st)) return rc;
+    if (nDet > 0) {
+      const int n = Bg * nDet;
+      hipLaunchKernelGGL(det_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, st, det_ws,
+                         a.strips_per_image, nDet, Bg, det_out + static_cast<long long>(g0) * nDet);
+      if (int rc = check_hip(hipGetLastError(), "det_finalize_kernel launch")) return rc;
+    }
   }
   return kOk;
 }

@@ -1,0 +1,62 @@
+// Row and column passes of the fused slice step, shared by the ABI drivers.
+#pragma once
+#include "common.hpp"
+
+namespace namsa {
+
+enum RowOp { ROW_STEP = 0, ROW_INV = 1, ROW_FWD = 2 };
+enum ColOp { COL_PROP = 0, COL_PROBE = 1, COL_READOUT = 2, COL_FWD = 3, COL_INV = 4 };
+enum Pass { PASS_ROW = 0, PASS_COL = 1 };
+
+// Row pass over rows of length N = Nx: psi [n_rows][N], tsl [rows_per_image][N].
+// Entries outside [band_lo, band_hi) are known to be zero and are not loaded.
+struct RowArgs {
+  float2* psi;
+  long long n_rows;
+  int rows_per_image;
+  const float2* tsl;  // STEP: transmission slice
+  float scale;        // INV: output scale
+  int band_lo, band_hi;
+};
+
+// Column pass over columns of length N = Ny of B images [B][Ny][Nx].  The
+// launcher covers strips [strip0, strip0 + n_strips) of width C; READOUT and
+// the transforms cover every strip, PROP only those meeting [band_lo, band_hi)
+// (outside that band the propagator, and so the result, is zero).
+struct ColArgs {
+  float2* psi;
+  int B;
+  int Nx;
+  int strips_per_image;  // Nx / C, filled in by the launcher
+  int strip0, n_strips;  // filled in by the launcher
+  int band_lo, band_hi;  // columns where pm may be nonzero
+  const float2* pm;      // [Ny][Nx] propagator * mask (PROP, READOUT)
+  float pm_scale;        // PROP: factor folded into pm (1/N^2)
+  const float2* psi_k;   // [Ny][Nx] (PROBE)
+  const float* pos;      // [B][2] (y, x) (PROBE)
+  double kmax;           // (PROBE)
+  float amp;             // intensity factor (READOUT)
+  int apply_pm;          // (READOUT)
+  float* cbed;           // [B][Ny][Nx] (READOUT)
+  const float* det;      // [n_det][Ny][Nx] (READOUT)
+  int n_det;
+  double* det_ws;        // [B][strips_per_image][n_det] (READOUT)
+};
+
+// Narrowest column tile of any compiled schedule: detector scratch is sized
+// for it so that any schedule choice fits.
+constexpr int kMinColTile = 4;
+
+bool supported(int n);
+int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st);
+// Launches the selected column schedule; on return a.strips_per_image holds
+// the strip count that was used.
+int col_pass(int Ny, int op, ColArgs& a, hipStream_t st);
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+}  // namespace namsa

@@ -1,0 +1,365 @@
+// Kernel templates of the row and column passes, parameterised by a schedule.
+//
+// A row schedule fixes the radix list and elements per thread E of the
+// x-transforms; rows of length N are handled by T = N/E threads, several rows
+// per 256-thread workgroup when T < 256.  A column schedule also fixes the
+// tile width C (adjacent columns per workgroup, C*8 contiguous bytes per row
+// segment).  PF prefetches the next row/tile of the persistent loop into
+// registers while the current one is transformed, so each wavefront keeps a
+// load stream in flight through its compute phase.  NT marks the wave-state
+// loads and stores non-temporal so the streamed state does not evict the
+// shared slice and propagator from the caches.
+#pragma once
+#include <algorithm>
+#include <type_traits>
+
+#include "fft.hpp"
+#include "passes.hpp"
+
+namespace namsa {
+
+constexpr int kRowThreads = 256;
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ float2 ld(const float2* p) {
+  if constexpr (NT) {
+    const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+    return make_float2(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+
+template <bool NT>
+__device__ __forceinline__ void st(float2* p, float2 v) {
+  if constexpr (NT) {
+    f2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+  } else {
+    *p = v;
+  }
+}
+
+template <int N_, int E_, class RL_, bool PF_, bool NT_>
+struct RowS {
+  static constexpr int N = N_, E = E_;
+  using RL = RL_;
+  static constexpr bool PF = PF_, NT = NT_;
+};
+
+template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false>
+struct ColS {
+  static constexpr int N = N_, E = E_, C = C_;
+  using RL = RL_;
+  static constexpr bool PF = PF_, NT = NT_, DB = DB_;
+  static constexpr int threads() { return C * (N / E); }
+};
+
+// exp(2 pi i * turns) with the phase reduced to [-1/2, 1/2] turns in double.
+__device__ __forceinline__ float2 unit_phase(double turns) {
+  const double f = turns - rint(turns);
+  double s, c;
+  sincospi(2.0 * f, &s, &c);
+  return make_float2(float(c), float(s));
+}
+
+// ------------------------------------------------------------------ row pass
+template <class S, int OP>
+__global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
+  constexpr int N = S::N, E = S::E;
+  using F = Fft<N, E, typename S::RL>;
+  constexpr int T = F::T;
+  constexpr int G = kRowThreads / T;
+  constexpr bool STEP = OP == ROW_STEP;
+  using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+  __shared__ float2 smem[G * RowLds::padded(N)];
+
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  RowLds lds{smem + g * RowLds::padded(N)};
+  float2 tw[F::NTW_ALLOC];
+  F::twiddles(t, tw);
+
+  const long long n_blocks = (a.n_rows + G - 1) / G;
+  auto clamp_row = [&](long long rb) {
+    const long long r = rb * G + g;
+    return r < a.n_rows ? r : a.n_rows - 1;
+  };
+  auto load = [&](long long r, float2(&v)[E], float2(&tv)[E]) {
+    const float2* p = a.psi + r * N + t;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int x = t + T * m;
+      v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + T * m) : make_float2(0.f, 0.f);
+    }
+    if constexpr (STEP) {
+      const float2* tr = a.tsl + static_cast<long long>(r % a.rows_per_image) * N + t;
+#pragma unroll
+      for (int m = 0; m < E; ++m) tv[m] = tr[T * m];
+    }
+  };
+
+  float2 pv[E], pt[E];
+  long long rb = blockIdx.x;
+  if constexpr (S::PF) {
+    if (rb < n_blocks) load(clamp_row(rb), pv, pt);
+  }
+  for (; rb < n_blocks; rb += gridDim.x) {
+    const long long row = rb * G + g;
+    const long long rr = clamp_row(rb);
+    float2 v[E], tv[E];
+    if constexpr (S::PF) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        v[m] = pv[m];
+        tv[m] = pt[m];
+      }
+      if (rb + gridDim.x < n_blocks) load(clamp_row(rb + gridDim.x), pv, pt);
+    } else {
+      load(rr, v, tv);
+    }
+    if constexpr (STEP) {
+      F::template run<true, Sync>(v, lds, t, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tv[m]);
+      F::template run<false, Sync>(v, lds, t, tw);
+    } else if constexpr (OP == ROW_INV) {
+      F::template run<true, Sync>(v, lds, t, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cscale(v[m], a.scale);
+    } else {
+      F::template run<false, Sync>(v, lds, t, tw);
+    }
+    if (row < a.n_rows) {
+      float2* p = a.psi + row * N + t;
+#pragma unroll
+      for (int m = 0; m < E; ++m) st<S::NT>(p + T * m, v[m]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ column pass
+template <class S, int OP>
+__global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
+  constexpr int N = S::N, E = S::E, C = S::C;
+  using F = Fft<N, E, typename S::RL>;
+  constexpr int T = F::T;
+  constexpr int NW = S::threads() / 64;
+  constexpr bool LOADS = OP != COL_PROBE;
+  using Lds = ColLds<C, S::DB>;
+  constexpr int LDS_ONE = Lds::padded(N);
+  __shared__ float2 smem[S::DB ? 2 * LDS_ONE : LDS_ONE];
+  __shared__ double red[NW > 0 ? NW : 1];
+
+  const int c = threadIdx.x % C;
+  const int jt = threadIdx.x / C;
+  Lds lds{smem, c, LDS_ONE};
+  float2 tw[F::NTW_ALLOC];
+  F::twiddles(jt, tw);
+  const long long plane = static_cast<long long>(N) * a.Nx;
+  const long long stride = a.Nx;
+
+  // Workgroup w owns column strip w % strips for the whole launch and visits
+  // images w / strips, w / strips + step, ... (the launcher makes gridDim.x a
+  // multiple of strips), so the strip's propagator and probe values are read
+  // once into registers; the workgroups resident at one time still cover all
+  // strips of a few images, which keeps HBM rows contiguous.
+  const int strip = a.strip0 + static_cast<int>(blockIdx.x % a.n_strips);
+  const int col = strip * C + c;
+  const long long step = gridDim.x / a.n_strips;
+  // READOUT covers every strip; a strip outside the band has a zero
+  // propagator, so its intensities are zero and its wave is never read.
+  const bool live = OP != COL_READOUT || !a.apply_pm ||
+                    (strip * C < a.band_hi && strip * C + C > a.band_lo);
+
+  float2 cst[E];  // PROP / READOUT: propagator * mask; PROBE: psi_k * (-1)^(row+col)
+  if constexpr (OP == COL_PROP || OP == COL_READOUT) {
+    if (OP == COL_PROP || a.apply_pm) {
+      const float sc = OP == COL_PROP ? a.pm_scale : 1.0f;
+#pragma unroll
+      for (int m = 0; m < E; ++m) cst[m] = cscale(a.pm[(jt + T * m) * stride + col], sc);
+    }
+  } else if constexpr (OP == COL_PROBE) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int row = jt + T * m;
+      const float2 k = a.psi_k[static_cast<long long>(row) * stride + col];
+      cst[m] = ((row + col) & 1) ? make_float2(-k.x, -k.y) : k;
+    }
+  }
+
+  auto load = [&](long long b, float2(&v)[E]) {
+    const float2* p = a.psi + b * plane + col;
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = ld<S::NT>(p + (jt + T * m) * stride);
+  };
+
+  float2 pv[E];
+  long long b = blockIdx.x / a.n_strips;
+  if constexpr (S::PF && LOADS) {
+    if (b < a.B && live) load(b, pv);
+  }
+  for (; b < a.B; b += step) {
+    float2* base = a.psi + b * plane + col;
+    float2 v[E];
+
+    if constexpr (OP == COL_PROBE) {
+      // psi_k * exp(2 pi i (-kx rx - ky ry)) * (-1)^(row + col)
+      // (build_probes_stack + fftshift_2d_stack, probe_kernels.cu:239-304)
+      const double ry = a.pos[2 * b];
+      const double rx = a.pos[2 * b + 1];
+      const float2 ex = unit_phase(-k_lin(col, a.kmax, a.Nx) * rx);
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        v[m] = cmul(cst[m], cmul(ex, unit_phase(-k_lin(jt + T * m, a.kmax, N) * ry)));
+      F::template run<true, BlockSync>(v, lds, jt, tw);
+    } else if (!live) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = make_float2(0.f, 0.f);
+    } else {
+      if constexpr (S::PF) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = pv[m];
+        if (b + step < a.B) load(b + step, pv);
+      } else {
+        load(b, v);
+      }
+      if constexpr (OP == COL_PROP) {
+        F::template run<false, BlockSync>(v, lds, jt, tw);
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cst[m]);
+        F::template run<true, BlockSync>(v, lds, jt, tw);
+      } else if constexpr (OP == COL_FWD || OP == COL_READOUT) {
+        F::template run<false, BlockSync>(v, lds, jt, tw);
+      } else if constexpr (OP == COL_INV) {
+        F::template run<true, BlockSync>(v, lds, jt, tw);
+      }
+    }
+
+    if constexpr (OP == COL_READOUT) {
+      float inten[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const long long off = (jt + T * m) * stride + col;
+        float2 y = v[m];
+        if (a.apply_pm) y = cmul(y, cst[m]);
+        inten[m] = (y.x * y.x + y.y * y.y) * a.amp;
+        a.cbed[b * plane + off] = inten[m];
+      }
+      for (int d = 0; d < a.n_det; ++d) {
+        const float* w = a.det + d * plane;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) s += double(inten[m]) * double(w[(jt + T * m) * stride + col]);
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double tot = 0.0;
+#pragma unroll
+          for (int i = 0; i < NW; ++i) tot += red[i];
+          a.det_ws[(b * a.strips_per_image + strip) * a.n_det + d] = tot;
+        }
+        __syncthreads();
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) st<S::NT>(base + (jt + T * m) * stride, v[m]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <class K>
+int persistent_cap(K kernel, int threads) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  return per_cu * device_cu_count();
+}
+
+template <class S, int OP>
+int launch_row_op(const RowArgs& a, hipStream_t st) {
+  auto k = row_kernel<S, OP>;
+  constexpr int G = kRowThreads / (S::N / S::E);
+  static const int cap = persistent_cap(k, kRowThreads);
+  const long long blocks = (a.n_rows + G - 1) / G;
+  const int grid = static_cast<int>(std::max<long long>(1, std::min<long long>(blocks, cap)));
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kRowThreads), 0, st, a);
+  return check_hip(hipGetLastError(), "row pass launch");
+}
+
+template <class S, int OP>
+int launch_col_op(ColArgs& a, hipStream_t st) {
+  auto k = col_kernel<S, OP>;
+  static const int cap = persistent_cap(k, S::threads());
+  a.strips_per_image = a.Nx / S::C;
+  if (OP == COL_PROP) {  // only the strips the propagator band reaches
+    a.strip0 = a.band_lo / S::C;
+    a.n_strips = (a.band_hi + S::C - 1) / S::C - a.strip0;
+  } else {
+    a.strip0 = 0;
+    a.n_strips = a.strips_per_image;
+  }
+  // a whole number of strip sets, so that every workgroup keeps one strip
+  const long long sets = std::max<long long>(1, std::min<long long>(a.B, cap / a.n_strips));
+  const int grid = static_cast<int>(sets * a.n_strips);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(S::threads()), 0, st, a);
+  return check_hip(hipGetLastError(), "column pass launch");
+}
+
+using RowLaunch = int (*)(int op, const RowArgs&, hipStream_t);
+using ColLaunch = int (*)(int op, ColArgs&, hipStream_t);
+
+struct RowSchedEntry {
+  const char* name;
+  RowLaunch launch;
+};
+struct ColSchedEntry {
+  const char* name;
+  ColLaunch launch;
+  int C;
+};
+struct SchedTable {
+  const RowSchedEntry* rows;
+  int n_rows;
+  const ColSchedEntry* cols;
+  int n_cols;
+};
+
+// The base schedule (entry 0 of a table) serves every op; the alternatives
+// only replace the hot ops (row STEP, column PROP and READOUT) and defer the
+// rest to the base schedule.
+template <class S, class Base>
+int launch_row(int op, const RowArgs& a, hipStream_t st) {
+  if (op == ROW_STEP) return launch_row_op<S, ROW_STEP>(a, st);
+  if constexpr (std::is_same<S, Base>::value) {
+    if (op == ROW_INV) return launch_row_op<S, ROW_INV>(a, st);
+    return launch_row_op<S, ROW_FWD>(a, st);
+  } else {
+    return launch_row<Base, Base>(op, a, st);
+  }
+}
+
+template <class S, class Base>
+int launch_col(int op, ColArgs& a, hipStream_t st) {
+  if (op == COL_PROP) return launch_col_op<S, COL_PROP>(a, st);
+  if (op == COL_READOUT) return launch_col_op<S, COL_READOUT>(a, st);
+  if constexpr (std::is_same<S, Base>::value) {
+    if (op == COL_PROBE) return launch_col_op<S, COL_PROBE>(a, st);
+    if (op == COL_FWD) return launch_col_op<S, COL_FWD>(a, st);
+    return launch_col_op<S, COL_INV>(a, st);
+  } else {
+    return launch_col<Base, Base>(op, a, st);
+  }
+}
+
+template <int N>
+const SchedTable* sched_table();
+
+}  // namespace namsa

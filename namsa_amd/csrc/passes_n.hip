@@ -1,0 +1,122 @@
+// Compiled schedules of the row and column passes for one transform length.
+// Built once per length with -DNAMSA_PASS_N=<n>; entry 0 is the base schedule.
+// Names read: e<E> = elements per thread, r<radices>, c<C> = column tile,
+// pf = next row/tile prefetched, nt = non-temporal wave traffic.
+#include "passes_impl.hpp"
+
+#ifndef NAMSA_PASS_N
+#error "compile with -DNAMSA_PASS_N=<transform length>"
+#endif
+
+namespace namsa {
+
+#if NAMSA_PASS_N == 128
+using R0 = RowS<128, 8, Radices<8, 4, 4>, false, false>;
+using R1 = RowS<128, 8, Radices<8, 4, 4>, true, false>;
+using C0 = ColS<128, 8, Radices<8, 4, 4>, 16, false, false>;
+using C1 = ColS<128, 8, Radices<8, 4, 4>, 8, true, false>;
+static const RowSchedEntry kRows[] = {{"e8.r844", launch_row<R0, R0>}, {"e8.r844.pf", launch_row<R1, R0>}};
+static const ColSchedEntry kCols[] = {{"e8.r844.c16", launch_col<C0, C0>, 16},
+                                      {"e8.r844.c8.pf", launch_col<C1, C0>, 8}};
+#elif NAMSA_PASS_N == 192
+using R0 = RowS<192, 12, Radices<3, 4, 4, 4>, false, false>;
+using R1 = RowS<192, 12, Radices<3, 4, 4, 4>, true, false>;
+using C0 = ColS<192, 12, Radices<3, 4, 4, 4>, 16, false, false>;
+using C1 = ColS<192, 12, Radices<3, 4, 4, 4>, 8, true, false>;
+static const RowSchedEntry kRows[] = {{"e12.r3444", launch_row<R0, R0>}, {"e12.r3444.pf", launch_row<R1, R0>}};
+static const ColSchedEntry kCols[] = {{"e12.r3444.c16", launch_col<C0, C0>, 16},
+                                      {"e12.r3444.c8.pf", launch_col<C1, C0>, 8}};
+#elif NAMSA_PASS_N == 256
+using R0 = RowS<256, 8, Radices<8, 8, 4>, false, false>;
+using R1 = RowS<256, 8, Radices<8, 8, 4>, true, false>;
+using R2 = RowS<256, 16, Radices<16, 16>, true, false>;
+using C0 = ColS<256, 8, Radices<8, 8, 4>, 16, false, false>;
+using C1 = ColS<256, 8, Radices<8, 8, 4>, 8, true, false>;
+using C2 = ColS<256, 16, Radices<16, 16>, 16, true, false>;
+static const RowSchedEntry kRows[] = {{"e8.r884", launch_row<R0, R0>},
+                                      {"e8.r884.pf", launch_row<R1, R0>},
+                                      {"e16.r16.16.pf", launch_row<R2, R0>}};
+static const ColSchedEntry kCols[] = {{"e8.r884.c16", launch_col<C0, C0>, 16},
+                                      {"e8.r884.c8.pf", launch_col<C1, C0>, 8},
+                                      {"e16.r16.16.c16.pf", launch_col<C2, C0>, 16}};
+#elif NAMSA_PASS_N == 384
+using R0 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, false, false>;
+using R1 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, true, false>;
+using C0 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 16, false, false>;
+using C1 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false>;
+static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>}, {"e12.r32444.pf", launch_row<R1, R0>}};
+static const ColSchedEntry kCols[] = {{"e12.r32444.c16", launch_col<C0, C0>, 16},
+                                      {"e12.r32444.c8.pf", launch_col<C1, C0>, 8}};
+#elif NAMSA_PASS_N == 512
+using R0 = RowS<512, 8, Radices<8, 8, 8>, false, false>;
+using R1 = RowS<512, 8, Radices<8, 8, 8>, true, false>;
+using R2 = RowS<512, 8, Radices<8, 8, 8>, true, true>;
+using R3 = RowS<512, 16, Radices<16, 16, 2>, false, false>;
+using R4 = RowS<512, 16, Radices<16, 16, 2>, true, false>;
+using C0 = ColS<512, 16, Radices<8, 8, 8>, 16, false, false>;
+using C1 = ColS<512, 8, Radices<8, 8, 8>, 8, false, false>;
+using C2 = ColS<512, 8, Radices<8, 8, 8>, 8, true, false>;
+using C3 = ColS<512, 8, Radices<8, 8, 8>, 16, false, false>;
+using C4 = ColS<512, 16, Radices<16, 16, 2>, 8, true, false>;
+using C5 = ColS<512, 8, Radices<8, 8, 8>, 8, true, true>;
+using C6 = ColS<512, 16, Radices<16, 16, 2>, 16, false, false>;
+using C7 = ColS<512, 8, Radices<8, 8, 8>, 16, false, false, true>;
+using C8 = ColS<512, 16, Radices<8, 8, 8>, 16, false, false, true>;
+static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
+                                      {"e8.r888.pf", launch_row<R1, R0>},
+                                      {"e8.r888.pf.nt", launch_row<R2, R0>},
+                                      {"e16.r16.16.2", launch_row<R3, R0>},
+                                      {"e16.r16.16.2.pf", launch_row<R4, R0>}};
+static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
+                                      {"e8.r888.c8", launch_col<C1, C0>, 8},
+                                      {"e8.r888.c8.pf", launch_col<C2, C0>, 8},
+                                      {"e8.r888.c16", launch_col<C3, C0>, 16},
+                                      {"e16.r16.16.2.c8.pf", launch_col<C4, C0>, 8},
+                                      {"e8.r888.c8.pf.nt", launch_col<C5, C0>, 8},
+                                      {"e16.r16.16.2.c16", launch_col<C6, C0>, 16},
+                                      {"e8.r888.c16.db", launch_col<C7, C0>, 16},
+                                      {"e16.r888.c16.db", launch_col<C8, C0>, 16}};
+#elif NAMSA_PASS_N == 768
+using R0 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false>;
+using R1 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, true, false>;
+using C0 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false>;
+using C1 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false>;
+static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>}, {"e12.r34444.pf", launch_row<R1, R0>}};
+static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
+                                      {"e12.r34444.c8.pf", launch_col<C1, C0>, 8}};
+#elif NAMSA_PASS_N == 1024
+using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
+using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
+using R2 = RowS<1024, 8, Radices<8, 8, 4, 4>, true, false>;
+using C0 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false>;
+using C1 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false>;
+using C2 = ColS<1024, 8, Radices<8, 8, 4, 4>, 8, false, false>;
+static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
+                                      {"e16.r16.8.8.pf", launch_row<R1, R0>},
+                                      {"e8.r8844.pf", launch_row<R2, R0>}};
+static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
+                                      {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
+                                      {"e8.r8844.c8", launch_col<C2, C0>, 8}};
+#elif NAMSA_PASS_N == 2048
+using R0 = RowS<2048, 16, Radices<16, 16, 8>, false, false>;
+using R1 = RowS<2048, 16, Radices<16, 16, 8>, true, false>;
+using C0 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false>;
+using C1 = ColS<2048, 16, Radices<16, 16, 8>, 8, false, false>;
+using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
+static const RowSchedEntry kRows[] = {{"e16.r16.16.8", launch_row<R0, R0>},
+                                      {"e16.r16.16.8.pf", launch_row<R1, R0>}};
+static const ColSchedEntry kCols[] = {{"e16.r16.16.8.c4", launch_col<C0, C0>, 4},
+                                      {"e16.r16.16.8.c8", launch_col<C1, C0>, 8},
+                                      {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4}};
+#else
+#error "no schedules for this transform length"
+#endif
+
+template <>
+const SchedTable* sched_table<NAMSA_PASS_N>() {
+  static const SchedTable table{kRows, static_cast<int>(sizeof(kRows) / sizeof(kRows[0])), kCols,
+                                static_cast<int>(sizeof(kCols) / sizeof(kCols[0]))};
+  return &table;
+}
+
+}  // namespace namsa

@@ -40,10 +40,28 @@ DEFAULT_PROBE = {"smooth_apert": True, "apert_smooth": 50, "spherical_phase": Tr
 GPU_APERTURE_STEEPNESS = 80.0  # fixed in soft_aperture, namsa/probe_kernels.cu:203-204
 
 
+LLC_BYTES = 256 << 20  # MI355X last-level (Infinity) cache
+
+
 def _torch():
     import torch
 
     return torch
+
+
+def cache_group(Ny, Nx):
+    """Probes whose complex64 waves fill ~3/4 of the last-level cache.
+
+    Pushing such a group through all slices before starting the next keeps the
+    slice loop's wave traffic in the cache instead of HBM.  The autotuner's
+    cached choice (namsa_amd.autotune.tune_group) overrides this default.
+    """
+    from . import autotune
+
+    g = autotune.cached_group(Ny, Nx)
+    if g:
+        return g
+    return max(1, int(LLC_BYTES * 3 // 4 // (8 * Ny * Nx)))
 
 
 class MSAGPU:
@@ -260,7 +278,8 @@ class MSAGPU:
         pos0 = torch.zeros((1, 2), dtype=torch.float32, device=dev)
         psi = torch.empty((1, Ny, Nx), dtype=torch.complex64, device=dev)
         call("namsa_multislice", ptr(self.psi_k_device), ptr(pos0), 1, Ny, Nx, np.float32(self.kmax),
-             None, 0, None, _lib.OUT_INCIDENT, np.float32(1.0), ptr(psi), None, None, 0, None, None, st)
+             None, 0, None, 0, Nx, 0, _lib.OUT_INCIDENT, np.float32(1.0), ptr(psi), None, None, 0,
+             None, None, st)
         norm = torch.empty(1, dtype=torch.float64, device=dev)
         call("namsa_norm2", ptr(psi), Ny * Nx, ptr(norm), st)
         self.normalization = float(norm.item())
@@ -323,10 +342,14 @@ class MSAGPU:
         call("namsa_propagator", Ny, Nx, np.float32(self.kmax), np.float32(bandwidth * self.kmax),
              np.float32(self.Lambda), np.float32(getattr(self, "slice_t", 0.0)),
              ptr(self.pm_device), ptr(self.mask_device), stream_handle())
+        # columns the bandwidth mask leaves open: outside them every slice's
+        # spectrum is exactly zero and the kernels skip the work
+        cols = np.nonzero(self.mask_device.cpu().numpy().any(axis=0))[0]
+        self.band = (int(cols[0]), int(cols[-1]) + 1) if len(cols) else (0, Nx)
         return self.pm_device
 
     def multislice(self, bandwidth=1 / 3, unified_mem=False, batch_size=None, transmit=True,
-                   detectors=None, output="auto"):
+                   detectors=None, output="auto", group=None):
         """Propagate every probe position through all slices (msa.py:767-924).
 
         transmit=True: ``probes`` = float32 CBED [P][Ny][Nx], normalised by the
@@ -335,6 +358,8 @@ class MSAGPU:
         exit waves / sqrt(normalization) instead.  ``detectors`` ([nDet][Ny][Nx]
         float weights) fuses the detector integrals into the readout; the
         result lands in ``self.detector_signals`` [P][nDet] (float64).
+        ``group`` probes go through all slices together (default: as many as
+        fit the last-level cache, see ``cache_group``).
         """
         torch = _torch()
         dev = self._dev()
@@ -361,9 +386,11 @@ class MSAGPU:
         chunk = min(int(self.max_probes), P)
         if batch_size:
             chunk = min(chunk, int(batch_size))
+        group = int(group) if group else cache_group(Ny, Nx)
+        ws_planes = min(chunk, group)
         pos_all = torch.from_numpy(np.ascontiguousarray(self.probe_positions, dtype=np.float32)).to(dev)
         if mode == _lib.OUT_CBED:
-            psi_ws = torch.empty((chunk, Ny, Nx), dtype=torch.complex64, device=dev)
+            psi_ws = torch.empty((ws_planes, Ny, Nx), dtype=torch.complex64, device=dev)
             out = torch.empty((P, Ny, Nx), dtype=torch.float32, device=dev)
             scale = np.float32(1.0 / self.normalization)
         else:
@@ -388,8 +415,8 @@ class MSAGPU:
                 cbed_ptr, ws_ptr = None, ptr(out[start])
             det_args = (ptr(det), nDet, ptr(det_ws), ptr(self._det_dev[start])) if nDet else (None, 0, None, None)
             call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
-                 np.float32(self.kmax), t_arg, S, ptr(self.pm_device), mode, scale, ws_ptr,
-                 cbed_ptr, *det_args, st)
+                 np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
+                 group, mode, scale, ws_ptr, cbed_ptr, *det_args, st)
         self._probes_dev = out
         self._probes_host = None
         self.detector_signals = self._det_dev.cpu().numpy() if nDet else None

@@ -44,9 +44,30 @@ def test_argument_errors_report_without_gpu():
     assert rc == -1 and b"null data" in lib.namsa_last_error()
     rc = lib.namsa_fft2d(ctypes.c_void_p(16), 1, 500, 512, 0, None)
     assert rc == -2 and b"500" in lib.namsa_last_error()
-    rc = lib.namsa_multislice(None, None, 0, 512, 512, 8.0, None, 0, None, 0, 1.0, None, None, None,
-                              0, None, None, None)
+    rc = lib.namsa_multislice(None, None, 0, 512, 512, 8.0, None, 0, None, 0, 0, 0, 0, 1.0, None, None,
+                              None, 0, None, None, None)
     assert rc == -1
+    rc = lib.namsa_slice_cols(ctypes.c_void_p(16), 1, 512, 512, ctypes.c_void_p(16), 300, 200, None)
+    assert rc == -1 and b"band" in lib.namsa_last_error()
     with pytest.raises(L.NamsaError):
         L.call("namsa_norm2", None, 0, None, None)
-    assert lib.namsa_det_workspace_doubles(4, 512, 512, 2) == 4 * (512 // 16) * 2
+    # sized for the narrowest column tile of any schedule (4 columns)
+    assert lib.namsa_det_workspace_doubles(4, 512, 512, 2) == 4 * (512 // 4) * 2
+    assert lib.namsa_det_workspace_doubles(4, 500, 512, 2) == 0
+
+
+def test_schedule_registry_host_side():
+    from namsa_amd import autotune
+
+    for n in (128, 192, 256, 384, 512, 768, 1024, 2048):
+        for pass_ in (0, 1):
+            names = autotune.schedules(n, pass_)
+            assert len(names) >= 2 and len(set(names)) == len(names)
+            keep = autotune.selected(n, pass_)
+            autotune.select(n, pass_, names[-1])
+            assert autotune.selected(n, pass_) == len(names) - 1
+            autotune.select(n, pass_, keep)
+    lib = L.load()
+    assert lib.namsa_sched_count(500, 0) == 0
+    assert lib.namsa_sched_name(512, 0, 99) is None
+    assert lib.namsa_sched_set(512, 1, 99) == -1

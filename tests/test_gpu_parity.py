@@ -262,14 +262,16 @@ def test_vacuum_closed_form_2048(gpu):
     ws = torch.empty((2, N, N), dtype=torch.complex64, device=gpu)
     st = L.stream_handle()
     L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos), 2, N, N, np.float32(sim.kmax), None, 0,
-           None, L.OUT_CBED, np.float32(1 / sim.normalization), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
+           None, 0, 0, 0, L.OUT_CBED, np.float32(1 / sim.normalization), L.ptr(ws), L.ptr(cb), None, 0, None,
+           None, st)
     expect = np.abs(psi_k) ** 2 / np.sum(np.abs(psi_k) ** 2)
     for i in range(2):
         assert rel_l2(cb[i].cpu().numpy(), expect) < 1e-5
     ones = torch.ones((3, N, N), dtype=torch.complex64, device=gpu)
     pm = sim.build_propagator(1 / 3)
     L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos), 2, N, N, np.float32(sim.kmax), L.ptr(ones), 3,
-           L.ptr(pm), L.OUT_CBED, np.float32(1 / sim.normalization), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
+           L.ptr(pm), sim.band[0], sim.band[1], 1, L.OUT_CBED, np.float32(1 / sim.normalization), L.ptr(ws),
+           L.ptr(cb), None, 0, None, None, st)
     masked = expect * (np.abs(oracle_pm(sim, 1.0, 1 / 3)) > 0)
     for i in range(2):
         assert rel_l2(cb[i].cpu().numpy(), masked) < 1e-5

@@ -1,0 +1,68 @@
+"""Every compiled row/column schedule, at every transform length, against the
+FP64 oracle: exit waves and CBED after three slices with random unit-modulus
+slices and a random propagator that is zero outside a column band not aligned
+to any strip width, run in probe groups of 2 (rel-L2 < 1e-5, FP32 rounding of
+eight transforms)."""
+import sys
+
+import numpy as np
+import pytest
+
+import namsa_amd._lib as L
+from namsa_amd import autotune
+from oracle import msa_oracle as O
+from tests.parity_util import rel_l2
+
+pytestmark = pytest.mark.gpu
+SIZES = [128, 192, 256, 384, 512, 768, 1024, 2048]
+
+
+def _case(N, seed):
+    rng = np.random.default_rng(seed)
+    B, S = 3, 3
+    psi_k = (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N
+    pos = rng.uniform(-5, 5, (B, 2)).astype(np.float32)
+    t = np.exp(1j * rng.uniform(0, 2 * np.pi, (S, N, N)))
+    band = (N // 5 + 3, N - N // 4 - 1)
+    pm = np.exp(1j * rng.uniform(0, 2 * np.pi, (N, N))) * (rng.random((N, N)) < 0.8)
+    pm[:, :band[0]] = 0
+    pm[:, band[1]:] = 0
+    kmax = np.float32(6.5)
+    exit_ref = O.multislice(O.probe_stack(psi_k, pos, kmax), t, pm)
+    cbed_ref = O.readout(exit_ref, 1.0)
+    return psi_k, pos, t, pm, band, kmax, exit_ref, cbed_ref
+
+
+def _run(torch, dev, N, psi_k, pos, t, pm, band, kmax, mode):
+    B, S = len(pos), len(t)
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex64)).to(dev)
+    pk, tt, pmd = c64(psi_k), c64(t), c64(pm)
+    p = torch.from_numpy(pos).to(dev)
+    ws = torch.empty((B, N, N), dtype=torch.complex64, device=dev)
+    cb = torch.empty((B, N, N), dtype=torch.float32, device=dev)
+    L.call("namsa_multislice", L.ptr(pk), L.ptr(p), B, N, N, kmax, L.ptr(tt), S, L.ptr(pmd), band[0], band[1],
+           2, mode, np.float32(1.0), L.ptr(ws), L.ptr(cb), None, 0, None, None, L.stream_handle())
+    return (ws if mode == L.OUT_EXIT else cb).cpu().numpy()
+
+
+@pytest.mark.parametrize("N", SIZES)
+def test_every_schedule_matches_oracle(gpu, N):
+    import torch
+
+    psi_k, pos, t, pm, band, kmax, exit_ref, cbed_ref = _case(N, N + 1)
+    keep = (autotune.selected(N, 0), autotune.selected(N, 1))
+    try:
+        for pass_ in (0, 1):
+            for v, name in enumerate(autotune.schedules(N, pass_)):
+                autotune.select(N, 0, v if pass_ == 0 else 0)
+                autotune.select(N, 1, v if pass_ == 1 else 0)
+                print(f"N={N} pass={pass_} schedule={name}", flush=True)
+                sys.stdout.flush()
+                ex = _run(torch, gpu, N, psi_k, pos, t, pm, band, kmax, L.OUT_EXIT)
+                cb = _run(torch, gpu, N, psi_k, pos, t, pm, band, kmax, L.OUT_CBED)
+                e1, e2 = rel_l2(ex, exit_ref), rel_l2(cb, cbed_ref)
+                print(f"   exit relL2={e1:.2e} cbed relL2={e2:.2e}", flush=True)
+                assert e1 < 1e-5 and e2 < 1e-5, (name, e1, e2)
+    finally:
+        autotune.select(N, 0, keep[0])
+        autotune.select(N, 1, keep[1])
