@@ -141,3 +141,80 @@ def random_amorphous_crystal(seed=2, lateral=64.0, thickness=300.0, density=0.06
     xyz[int(np.argmax(xyz[:, 2])), 2] = thickness
     Z = rng.choice(np.asarray(zset), size=len(xyz))
     return make_records(xyz, Z)
+
+
+# ---------------------------------------------------------------- multi-material batch
+_LATTICES = ("cubic", "tetragonal", "hexagonal")
+
+
+def random_crystal(rng, lateral, thickness, a_range=(3.0, 7.0), n_basis=(1, 8), z_range=(3, 83),
+                   min_sep=1.0):
+    """One seeded random crystal tiled to lateral x lateral x thickness Angstrom.
+
+    Lattice: cubic (a, a, a), tetragonal (a, a, c) or hexagonal (a, a, c at
+    120 degrees, tiled through its orthohexagonal cell a x a*sqrt(3) x c, two
+    lattice points per cell), with a and c in a_range.  The basis has n_basis
+    atoms at random fractional positions kept >= min_sep apart (periodic
+    images included), each with Z uniform in z_range.
+    """
+    kind = _LATTICES[int(rng.integers(len(_LATTICES)))]
+    a = float(rng.uniform(*a_range))
+    c = a if kind == "cubic" else float(rng.uniform(*a_range))
+    if kind == "hexagonal":
+        cell = np.array([a, a * np.sqrt(3), c])
+        centring = np.array([[0, 0, 0], [0.5, 0.5, 0]])
+    else:
+        cell = np.array([a, a, c])
+        centring = np.zeros((1, 3))
+    nb = int(rng.integers(n_basis[0], n_basis[1] + 1))
+    basis = []
+    tries = 0
+    while len(basis) < nb and tries < 2000:
+        tries += 1
+        f = rng.random(3)
+        pts = (f + centring) % 1.0
+        ok = True
+        for q in basis:
+            for p in pts:
+                d = (p - q + 0.5) % 1.0 - 0.5
+                if np.linalg.norm(d * cell) < min_sep:
+                    ok = False
+        if ok and all(np.linalg.norm(((pts[i] - pts[j] + 0.5) % 1.0 - 0.5) * cell) >= min_sep
+                      for i in range(len(pts)) for j in range(i)):
+            basis.append(f)
+    basis = np.array(basis)
+    Zb = rng.integers(z_range[0], z_range[1] + 1, size=len(basis))
+    frac = ((basis[:, None, :] + centring[None]) % 1.0).reshape(-1, 3)
+    Zc = np.repeat(Zb, len(centring))
+    reps = np.ceil(np.array([lateral, lateral, thickness]) / cell).astype(int)
+    grid = np.array(np.meshgrid(*[np.arange(r) for r in reps], indexing="ij")).reshape(3, -1).T
+    xyz = ((grid[:, None, :] + frac[None]) * cell).reshape(-1, 3)
+    Z = np.tile(Zc, len(grid))
+    keep = (xyz[:, 0] < lateral) & (xyz[:, 1] < lateral) & (xyz[:, 2] < thickness)
+    return make_records(xyz[keep], Z[keep]), {"lattice": kind, "a": a, "c": c, "basis": len(basis)}
+
+
+def multi_material_specs(seed=3, count=48, grids=(128, 192, 256, 384, 512, 768, 1024),
+                         thickness=(50.0, 500.0), dz=(1.0, 3.0), kv=(60.0, 200.0),
+                         semi_mrad=(10.0, 30.0), probes=(1, 256), pix=(0.08, 0.14)):
+    """Seeded specifications of an uneven multi-material batch.
+
+    Each entry: records, grid N, dz, kV, semi-angle (rad), number of probe
+    positions (a square-ish raster) and the crystal description.  The lateral
+    size is N * pix with pix uniform in `pix`, so every grid samples its own
+    crystal at 0.08-0.14 Angstrom per pixel.
+    """
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        N = int(rng.choice(grids))
+        lateral = N * float(rng.uniform(*pix))
+        T = float(rng.uniform(*thickness))
+        rec, info = random_crystal(rng, lateral, T)
+        n_probe = int(rng.integers(probes[0], probes[1] + 1))
+        gx = int(np.floor(np.sqrt(n_probe)))
+        gy = int(np.ceil(n_probe / gx))
+        specs.append({"name": f"crystal{i:02d}", "records": rec, "N": N, "dz": float(rng.uniform(*dz)),
+                      "kv": float(rng.uniform(*kv)), "semi": float(rng.uniform(*semi_mrad)) * 1e-3,
+                      "grid_steps": (gx, gy), "thickness": T, "lateral": lateral, **info})
+    return specs
