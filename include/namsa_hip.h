@@ -89,6 +89,15 @@ int namsa_build_slices(const float* tables, int nZ, int Py, int Px, const int* s
                        void* t_out, void* stream);
 
 /*
+ * Projected potential V[S][Ny][Nx] float32 of the same site lists (the
+ * accumulation of namsa_build_slices without the exp(i sigma V) step); the
+ * corrector network takes the projected potential as an input channel.
+ */
+int namsa_build_potential(const float* tables, int nZ, int Py, int Px, const int* sites,
+                          const int* slice_offsets, int S, int Ny, int Nx, float* v_out,
+                          void* stream);
+
+/*
  * Probe in k-space psi_k = A(k) exp(-i chi(k)) on the linspace k-grid
  * (replaces soft_aperture / hard_aperture / spherical_phase_error /
  * mult_wise_c2d_re2d, namsa/probe_kernels.cu:180-222 and :126-134, launched at
@@ -189,6 +198,69 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
                      const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
                      int mode, float out_scale, void* psi_ws, float* cbed, const float* det,
                      int nDet, double* det_ws, double* det_out, void* stream);
+
+/*
+ * Row transforms of planes in place: forward (unscaled, scale must be 1) or
+ * inverse times scale.  The slice state between steps is the row transform
+ * of the wave, so the inverse with scale 1/Nx gives the wave in real space
+ * and the forward takes a real-space wave back to the state.  No counterpart
+ * in the reference (its state is the real-space wave, msa.py:971-981); the
+ * coarse-slice corrector uses it to read and correct the wave.
+ */
+int namsa_row_fft(void* data, int B, int Ny, int Nx, int inverse, float scale, void* stream);
+
+/*
+ * ---- Coarse-slice corrector network (no counterpart in the reference) ----
+ * Strided batched GEMM on the matrix cores, BF16 operands, FP32 accumulate:
+ *   C[b](i,j) = alpha sum_k A[b](i,k) B[b](k,j) + beta C[b](i,j) (+ bias, GELU)
+ * A(i,k) at A + b*sAb + i*sAi + k*sAk (FP32), likewise B(k,j) and C(i,j).
+ * split_k > 1 splits K over workgroups and adds into C with FP32 atomics
+ * (bias and GELU must then be off).  bias_mode 1: + bias[i], 2: + bias[j].
+ * gelu: C = gelu(value) and P (same strides as C) = value.
+ */
+int namsa_gemm_bf16(const float* A, long long sAb, long long sAi, long long sAk, const float* B,
+                    long long sBb, long long sBk, long long sBj, float* C, long long sCb,
+                    long long sCi, long long sCj, int M, int N, int K, int batch, int split_k,
+                    float alpha, float beta, const float* bias, int bias_mode, int gelu, float* P,
+                    void* stream);
+
+/* x[0..n) = v */
+int namsa_fill(float* x, long long n, float v, void* stream);
+/* g *= gelu'(pre), tanh form of GELU */
+int namsa_gelu_backward(float* g, const float* pre, long long n, void* stream);
+/* Adam with decoupled weight decay; step counts from 1 */
+int namsa_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+               float eps, int step, float wd, void* stream);
+/* u[cin][B*Ny*Nx] = (re, im) psi_c * s; with derivs also (re, im) of the
+ * periodic 5-point Laplacian, central y and central x differences of
+ * psi_c * s (6 channels); then V_j * sv for j < nv.  cin = 2 + 6 derivs + nv.
+ * V is [B][nv][Ny*Nx], or [nv][Ny*Nx] shared by every sample if v_shared. */
+int namsa_corrector_pack(const void* psi_c, const float* V, int nv, int v_shared, int derivs, float* u,
+                         int B, int Ny, int Nx, float s, float sv, void* stream);
+/* out[i] = |F[i]|^2 * scale (* mask[i % hw] if mask) over n complex values */
+int namsa_intensity(const void* F, long long n, long long hw, float scale, const float* mask, float* out,
+                    void* stream);
+/* per sample: sums[2b] = |out - (psi_f - psi_c) s|^2, sums[2b+1] = |(psi_f - psi_c) s|^2;
+ * gout (optional) = gradient of mean_b sums[2b] / sums[2b+1] */
+int namsa_corrector_loss(const float* out, const void* psi_c, const void* psi_f, int B, int hw, float s,
+                         double* sums, float* gout, void* stream);
+/*
+ * Mode bookkeeping of the spectral layers (m retained modes per axis, C
+ * channels, B samples).  combine: per (c,b) the 2m x 2m product
+ * P[(r,p)][(t,q)] of [cos; sin](y, p) with the x-transformed rows
+ * Z[y][(t,q)] = (sum x cos, -sum x sin) becomes
+ * Xs[p*m+q][(0,c)][b] = P[0p][0q] + P[1p][1q] (real part) and
+ * Xs[p*m+q][(1,c)][b] = P[0p][1q] - P[1p][0q] (imaginary part).
+ * expand: Ys[p*m+q][(r,o)][b] becomes, per (o,b), the 2m x 2m operand
+ * E[(r,p)][(t,q)] with E[0p][0q] = Yr, E[1p][0q] = -Yi, E[0p][1q] = Yi,
+ * E[1p][1q] = Yr, so that [cos | sin](y, (r,p)) E gives the y-inverse
+ * transform (real, imaginary) in columns (t,q).
+ */
+int namsa_modes_combine(const float* P, float* Xs, int C, int B, int m, void* stream);
+int namsa_modes_expand(const float* Ys, float* E, int C, int B, int m, void* stream);
+/* psi = psi_c + out / s (out is [2][B*hw]) */
+int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B, int hw, float s,
+                          void* stream);
 
 #ifdef __cplusplus
 }

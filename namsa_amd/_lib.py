@@ -49,6 +49,19 @@ SIGNATURES = {
     "namsa_probe_init": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _vp]),
     "namsa_slice_rows": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
     "namsa_slice_cols": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
+    "namsa_build_potential": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "namsa_gemm_bf16": (_i, [_vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _i, _i, _i, _i, _i,
+                             _f, _f, _vp, _i, _i, _vp, _vp]),
+    "namsa_fill": (_i, [_vp, _ll, _f, _vp]),
+    "namsa_gelu_backward": (_i, [_vp, _vp, _ll, _vp]),
+    "namsa_adam": (_i, [_vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _f, _vp]),
+    "namsa_corrector_pack": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _i, _i, _f, _f, _vp]),
+    "namsa_intensity": (_i, [_vp, _ll, _ll, _f, _vp, _vp, _vp]),
+    "namsa_corrector_loss": (_i, [_vp, _vp, _vp, _i, _i, _f, _vp, _vp, _vp]),
+    "namsa_corrector_apply": (_i, [_vp, _vp, _vp, _i, _i, _f, _vp]),
+    "namsa_row_fft": (_i, [_vp, _i, _i, _i, _i, _f, _vp]),
+    "namsa_modes_combine": (_i, [_vp, _vp, _i, _i, _i, _vp]),
+    "namsa_modes_expand": (_i, [_vp, _vp, _i, _i, _i, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,544 @@
+"""Learned coarse-slice corrector on MI355X (no counterpart in the reference).
+
+A thick specimen costs one fused slice step per dz = 1 A of thickness.  The
+corrector takes coarse steps of `factor` (8) fine slices -- one transmission
+exp(i sigma V_coarse) and one propagator over factor*dz -- and corrects the
+wave after each coarse step with a Fourier neural operator trained on pairs
+(coarse step, `factor` fine steps) from the same incoming wave:
+
+    input  u = (Re, Im of s x_c, s Lap x_c, s d_y x_c, s d_x x_c, sigma V_0 .. sigma V_{factor-1})
+    output Delta (2 channels):  x <- x_c + Delta / s
+
+x_c is the real-space wave after the coarse step (H x W), Lap and d_y, d_x
+its periodic 5-point Laplacian and central differences (the splitting error
+of a coarse step is, to first order, a sum of sub-slice potentials times
+derivatives of the wave, and 1x1 layers cannot form derivatives), V_j the
+projected potentials of the fine slices the coarse slice merges (the network
+needs their order along z: that ordering is exactly what the single coarse
+transmission loses) and s = Nx sqrt(H W / norm) puts the wave at unit RMS.
+
+Network (the FP64 definition and hand-derived backward live in the oracle and
+are pinned there by finite differences): lift (2+factor -> C), L Fourier
+layers  x <- gelu(Re IDFT(G(k) [Re; Im] DFT(x)) + W x + b)  over m x m centred
+modes with a real 2C x 2C weight per mode, and a 2-layer pointwise head.
+Every product -- the truncated DFTs along y and x, the per-mode channel
+mixing, the pointwise layers and all weight gradients -- is one call of the
+BF16 MFMA GEMM (namsa_gemm_bf16, FP32 accumulation); GELU and biases run in
+its epilogue, the rest in the pointwise kernels of the same library.
+
+Data layout (HBM, FP32): activations [C][B*H*W] (channel-major, so a
+pointwise layer is one [C x C] x [C x n] GEMM and the x-DFT is one
+[(C B H) x W] x [W x 2m] GEMM); spectral coefficients [m*m][2C][B] (mode-major,
+so the mixing is one batched [2C x 2C] x [2C x B] GEMM over the m*m modes).
+
+Fallback: the predicted correction's relative size |Delta|^2 / |s x_c|^2 is a
+free by-product of the forward pass; when it exceeds `tau` for any probe of
+the batch, the coarse slice is redone with `factor` fine steps from the saved
+incoming state.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import NamsaError, call, ptr, stream_handle
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _addr(t, off=0):
+    return t.data_ptr() + 4 * int(off)
+
+
+class FnoNet:
+    """Parameters, forward and backward of the corrector network on the GPU."""
+
+    def __init__(self, H, W, cin=16, C=32, L=4, m=16, Hq=64, seed=0, device="cuda", derivs=True):
+        torch = _torch()
+        if m % 2 or m > min(H, W):
+            raise ValueError("m must be even and at most the grid size")
+        self.H, self.W, self.cin, self.C, self.L, self.m, self.Hq = H, W, cin, C, L, m, Hq
+        self.derivs = bool(derivs)      # inputs carry the wave's Laplacian and gradient
+        self.device = torch.device(device)
+        self.shapes = {
+            "Wl": (C, cin), "bl": (C,), "G": (L, m, m, 2 * C, 2 * C), "W": (L, C, C), "b": (L, C),
+            "Q1": (Hq, C), "bq1": (Hq,), "Q2": (2, Hq), "bq2": (2,),
+        }
+        self.offsets = {}
+        total = 0
+        for k, shp in self.shapes.items():
+            self.offsets[k] = total
+            total += int(np.prod(shp))
+            total = (total + 63) // 64 * 64          # 256-byte aligned groups
+        self.n_params = total
+        self.theta = torch.zeros(total, dtype=torch.float32, device=self.device)
+        self.grad = torch.zeros_like(self.theta)
+        self.adam_m = torch.zeros_like(self.theta)
+        self.adam_v = torch.zeros_like(self.theta)
+        self.adam_step = 0
+        self.init_params(seed)
+        k = np.arange(m) - m // 2
+        ty = 2 * np.pi * np.outer(k, np.arange(H)) / H              # [p][y]
+        tx = 2 * np.pi * np.outer(np.arange(W), k) / W              # [x][q]
+        fy = np.concatenate([np.cos(ty), np.sin(ty)], 0)             # [(r,p)][y]
+        fx = np.concatenate([np.cos(tx), -np.sin(tx)], 1)            # [x][(t,q)]
+        self.Fy = torch.from_numpy(fy.astype(np.float32)).to(self.device)
+        self.Fx = torch.from_numpy(fx.astype(np.float32)).to(self.device)
+        self.B = 0
+        self.st = stream_handle()
+
+    # ------------------------------------------------------------------ parameters
+    def init_params(self, seed=0):
+        rng = np.random.default_rng(seed)
+        fan = {"Wl": self.cin, "G": 2 * self.C, "W": self.C, "Q1": self.C, "Q2": self.Hq}
+        p = {}
+        for k, shp in self.shapes.items():
+            p[k] = rng.standard_normal(shp) / np.sqrt(fan[k]) if k in fan else np.zeros(shp)
+        # the output layer starts at zero: the network begins as "no correction"
+        p["Q2"][:] = 0.0
+        self.load(p)
+
+    def view(self, name):
+        shp = self.shapes[name]
+        o = self.offsets[name]
+        return self.theta[o:o + int(np.prod(shp))].view(shp)
+
+    def gview(self, name):
+        shp = self.shapes[name]
+        o = self.offsets[name]
+        return self.grad[o:o + int(np.prod(shp))].view(shp)
+
+    def load(self, params):
+        torch = _torch()
+        for k, shp in self.shapes.items():
+            a = np.asarray(params[k], dtype=np.float32).reshape(shp)
+            self.view(k).copy_(torch.from_numpy(a))
+
+    def params_numpy(self):
+        return {k: self.view(k).cpu().numpy().astype(np.float64) for k in self.shapes}
+
+    def grads_numpy(self):
+        return {k: self.gview(k).cpu().numpy().astype(np.float64) for k in self.shapes}
+
+    def state_dict(self):
+        return {"config": dict(H=self.H, W=self.W, cin=self.cin, C=self.C, L=self.L, m=self.m, Hq=self.Hq,
+                               derivs=self.derivs),
+                "theta": self.theta.detach().cpu()}
+
+    def _p(self, name, idx=0):
+        o = self.offsets[name]
+        if idx:
+            o += idx * int(np.prod(self.shapes[name][1:]))
+        return _addr(self.theta, o)
+
+    def _g(self, name, idx=0):
+        o = self.offsets[name]
+        if idx:
+            o += idx * int(np.prod(self.shapes[name][1:]))
+        return _addr(self.grad, o)
+
+    # ------------------------------------------------------------------ buffers
+    def ensure(self, B):
+        """Activation buffers for batches of up to B samples."""
+        if B <= self.B:
+            return
+        torch = _torch()
+        if self.C * B > 65535:
+            raise NamsaError("batch too large for the batched mode transforms")
+        H, W, C, L, m, Hq = self.H, self.W, self.C, self.L, self.m, self.Hq
+        n = B * H * W
+        f = dict(dtype=torch.float32, device=self.device)
+        self.u = torch.empty((self.cin, n), **f)
+        self.xs = [torch.empty((C, n), **f) for _ in range(L + 1)]
+        self.pre = [torch.empty((C, n), **f) for _ in range(L - 1)]
+        self.Xs = torch.empty((L, m * m, 2 * C, B), **f)
+        self.Z = torch.empty((C * B * H, 2 * m), **f)
+        self.Pm = torch.empty((C * B, 2 * m, 2 * m), **f)
+        self.Ys = torch.empty((m * m, 2 * C, B), **f)
+        self.gYs = torch.empty((m * m, 2 * C, B), **f)
+        self.E = torch.empty((C * B, 2 * m, 2 * m), **f)
+        self.Zp = torch.empty((C * B * H, 2 * m), **f)
+        self.hpre = torch.empty((Hq, n), **f)
+        self.h = torch.empty((Hq, n), **f)
+        self.out = torch.empty((2, n), **f)
+        self.gout = torch.empty((2, n), **f)
+        self.ga = torch.empty((C, n), **f)
+        self.gb = torch.empty((C, n), **f)
+        self.gh = torch.empty((Hq, n), **f)
+        self.ones = torch.empty(n, **f)
+        call("namsa_fill", ptr(self.ones), n, np.float32(1.0), self.st)
+        self.sums = torch.empty(2 * B, dtype=torch.float64, device=self.device)
+        self.B = B
+
+    # ------------------------------------------------------------------ GEMM
+    def _gemm(self, A, sA, Bm, sB, Cm, sC, M, N, K, batch=1, split=1, alpha=1.0, beta=0.0, bias=None,
+              bias_mode=0, gelu=False, P=None):
+        call("namsa_gemm_bf16", A, *sA, Bm, *sB, Cm, *sC, M, N, K, batch, split, np.float32(alpha),
+             np.float32(beta), bias, bias_mode, 1 if gelu else 0, P, self.st)
+
+    def _split(self, K):
+        return max(1, min(65535, K // 4096))
+
+    def _spectral_forward(self, x, Xs, B, alpha):
+        """Xs[pq][(r,c)][b] = alpha * (Re, Im) of the centred m x m DFT modes of x [C][B*H*W]."""
+        H, W, C, m = self.H, self.W, self.C, self.m
+        m2, CB = 2 * m, C * B
+        self._gemm(x, (0, W, 1), _addr(self.Fx), (0, m2, 1), _addr(self.Z), (0, m2, 1),
+                   CB * H, m2, W, alpha=alpha)
+        self._gemm(_addr(self.Fy), (0, H, 1), _addr(self.Z), (H * m2, m2, 1), _addr(self.Pm),
+                   (m2 * m2, m2, 1), m2, m2, H, batch=CB)
+        call("namsa_modes_combine", ptr(self.Pm), Xs, C, B, m, self.st)
+
+    def _spectral_inverse(self, Ys, dest, B, alpha, beta, gelu=False, P=None):
+        """dest = alpha * Re(sum_modes Y e^{+i k.r}) + beta * dest (GELU optional)."""
+        H, W, C, m = self.H, self.W, self.C, self.m
+        m2, CB = 2 * m, C * B
+        call("namsa_modes_expand", Ys, ptr(self.E), C, B, m, self.st)
+        self._gemm(_addr(self.Fy), (0, 1, H), _addr(self.E), (m2 * m2, m2, 1), _addr(self.Zp),
+                   (H * m2, m2, 1), H, m2, m2, batch=CB)
+        self._gemm(_addr(self.Zp), (0, m2, 1), _addr(self.Fx), (0, 1, m2), dest, (0, W, 1),
+                   CB * H, W, m2, alpha=alpha, beta=beta, gelu=gelu, P=P)
+
+    # ------------------------------------------------------------------ passes
+    def forward(self, B):
+        """out[2][B*H*W] from u[cin][B*H*W] (filled by namsa_corrector_pack)."""
+        self.ensure(B)
+        H, W, C, L, m, Hq = self.H, self.W, self.C, self.L, self.m, self.Hq
+        n = B * H * W
+        C2 = 2 * C
+        self._gemm(self._p("Wl"), (0, self.cin, 1), ptr(self.u), (0, n, 1), ptr(self.xs[0]), (0, n, 1),
+                   C, n, self.cin, bias=self._p("bl"), bias_mode=1)
+        for l in range(L):
+            x = ptr(self.xs[l])
+            Xs = _addr(self.Xs, l * m * m * C2 * B)
+            self._spectral_forward(x, Xs, B, 1.0)
+            self._gemm(self._p("G", l), (C2 * C2, C2, 1), Xs, (C2 * B, B, 1), ptr(self.Ys), (C2 * B, B, 1),
+                       C2, B, C2, batch=m * m)
+            self._gemm(self._p("W", l), (0, C, 1), x, (0, n, 1), ptr(self.xs[l + 1]), (0, n, 1), C, n, C,
+                       bias=self._p("b", l), bias_mode=1)
+            last = l == L - 1
+            self._spectral_inverse(ptr(self.Ys), ptr(self.xs[l + 1]), B, 1.0 / (H * W), 1.0,
+                                   gelu=not last, P=None if last else ptr(self.pre[l]))
+        self._gemm(self._p("Q1"), (0, C, 1), ptr(self.xs[L]), (0, n, 1), ptr(self.h), (0, n, 1), Hq, n, C,
+                   bias=self._p("bq1"), bias_mode=1, gelu=True, P=ptr(self.hpre))
+        self._gemm(self._p("Q2"), (0, Hq, 1), ptr(self.h), (0, n, 1), ptr(self.out), (0, n, 1), 2, n, Hq,
+                   bias=self._p("bq2"), bias_mode=1)
+        return self.out
+
+    def backward(self, B):
+        """grad = d(sum out * gout)/d theta for the last forward of B samples."""
+        H, W, C, L, m, Hq = self.H, self.W, self.C, self.L, self.m, self.Hq
+        n = B * H * W
+        C2 = 2 * C
+        sp = self._split(n)
+        call("namsa_fill", ptr(self.grad), self.n_params, np.float32(0.0), self.st)
+        go, ones = ptr(self.gout), ptr(self.ones)
+        self._gemm(go, (0, n, 1), ptr(self.h), (0, 1, n), self._g("Q2"), (0, Hq, 1), 2, Hq, n, split=sp)
+        self._gemm(go, (0, n, 1), ones, (0, 1, 0), self._g("bq2"), (0, 1, 1), 2, 1, n, split=sp)
+        self._gemm(self._p("Q2"), (0, 1, Hq), go, (0, n, 1), ptr(self.gh), (0, n, 1), Hq, n, 2)
+        call("namsa_gelu_backward", ptr(self.gh), ptr(self.hpre), Hq * n, self.st)
+        gh = ptr(self.gh)
+        self._gemm(gh, (0, n, 1), ptr(self.xs[L]), (0, 1, n), self._g("Q1"), (0, C, 1), Hq, C, n, split=sp)
+        self._gemm(gh, (0, n, 1), ones, (0, 1, 0), self._g("bq1"), (0, 1, 1), Hq, 1, n, split=sp)
+        ga, gb = self.ga, self.gb
+        self._gemm(self._p("Q1"), (0, 1, C), gh, (0, n, 1), ptr(ga), (0, n, 1), C, n, Hq)
+        for l in range(L - 1, -1, -1):
+            if l < L - 1:
+                call("namsa_gelu_backward", ptr(ga), ptr(self.pre[l]), C * n, self.st)
+            g = ptr(ga)
+            x = ptr(self.xs[l])
+            Xs = _addr(self.Xs, l * m * m * C2 * B)
+            self._gemm(g, (0, n, 1), ones, (0, 1, 0), self._g("b", l), (0, 1, 1), C, 1, n, split=sp)
+            self._gemm(g, (0, n, 1), x, (0, 1, n), self._g("W", l), (0, C, 1), C, C, n, split=sp)
+            # (dL/dRe Y, dL/dIm Y) = (Re, Im) DFT(gpre) / (H W)
+            self._spectral_forward(g, ptr(self.gYs), B, 1.0 / (H * W))
+            self._gemm(ptr(self.gYs), (C2 * B, B, 1), Xs, (C2 * B, 1, B), self._g("G", l), (C2 * C2, C2, 1),
+                       C2, C2, B, batch=m * m)
+            self._gemm(self._p("G", l), (C2 * C2, 1, C2), ptr(self.gYs), (C2 * B, B, 1), ptr(self.Ys),
+                       (C2 * B, B, 1), C2, B, C2, batch=m * m)
+            # dL/dx = Re(sum gX e^{+i k.r}) + W^T gpre
+            self._spectral_inverse(ptr(self.Ys), ptr(gb), B, 1.0, 0.0)
+            self._gemm(self._p("W", l), (0, 1, C), g, (0, n, 1), ptr(gb), (0, n, 1), C, n, C, beta=1.0)
+            ga, gb = gb, ga
+        g = ptr(ga)
+        self._gemm(g, (0, n, 1), ptr(self.u), (0, 1, n), self._g("Wl"), (0, self.cin, 1), C, self.cin, n,
+                   split=sp)
+        self._gemm(g, (0, n, 1), ones, (0, 1, 0), self._g("bl"), (0, 1, 1), C, 1, n, split=sp)
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.adam_step += 1
+        call("namsa_adam", ptr(self.theta), ptr(self.grad), ptr(self.adam_m), ptr(self.adam_v), self.n_params,
+             np.float32(lr), np.float32(b1), np.float32(b2), np.float32(eps), self.adam_step, np.float32(wd),
+             self.st)
+
+    def flops(self, B):
+        """Multiply-add flops (x2) of one forward pass on B samples."""
+        H, W, C, L, m, Hq = self.H, self.W, self.C, self.L, self.m, self.Hq
+        n = B * H * W
+        m2 = 2 * m
+        spec = 2 * (C * B * H * W * m2 + C * B * m2 * m2 * H)            # y/x DFT
+        spec += 2 * (m * m * 2 * C * 2 * C * B)                            # mixing
+        spec += 2 * (C * B * H * m2 * m2 + C * B * H * m2 * W)             # inverse
+        return 2 * n * C * self.cin + L * (spec + 2 * n * C * C) + 2 * n * (Hq * C + 2 * Hq)
+
+
+# ---------------------------------------------------------------------- material
+class CoarseFineMaterial:
+    """One structure prepared for fine (dz) and coarse (factor*dz) stepping.
+
+    Fine slices t_f[S_f], their projected potentials V_f[S_f] (float32), coarse
+    slices t_c[S_c] (the same site table binned factor*dz thick, so coarse
+    slice k merges fine slices factor*k .. factor*k + factor - 1) and both
+    propagators.
+    """
+
+    def __init__(self, sim, dz=1.0, factor=8, bandwidth=1 / 3, probe_dict=None):
+        torch = _torch()
+        self.sim = sim
+        self.factor = int(factor)
+        sim._check_grid()
+        if not hasattr(sim, "tables_device"):
+            sim.calc_atomic_potentials()
+        Ny, Nx = sim._grid
+        self.Ny, self.Nx = Ny, Nx
+        sim.build_potential_slices(dz)
+        self.t_f = sim.potential_slices_device
+        self.S_f = int(sim.num_slices)
+        sites, offs = sim.site_table(dz)
+        Py, Px = sim.table_shape
+        self.V_f = torch.empty((self.S_f, Ny, Nx), dtype=torch.float32, device=sim.device)
+        sites_d = torch.from_numpy(sites.reshape(-1)).to(sim.device) if len(sites) else None
+        call("namsa_build_potential", ptr(sim.tables_device), len(sim.unique_Z), Py, Px, ptr(sites_d),
+             ptr(torch.from_numpy(offs).to(sim.device)), self.S_f, Ny, Nx, ptr(self.V_f), stream_handle())
+        sim.slice_t = float(dz)
+        self.pm_f = sim.build_propagator(bandwidth).clone()
+        sim.build_potential_slices(dz * factor)
+        self.t_c = sim.potential_slices_device
+        self.S_c = int(sim.num_slices)
+        if self.S_c * factor > self.S_f:
+            raise NamsaError("coarse slices extend past the fine ones")
+        sim.slice_t = float(dz * factor)
+        self.pm_c = sim.build_propagator(bandwidth).clone()
+        self.mask = sim.mask_device.clone()
+        self.band = sim.band
+        if not hasattr(sim, "psi_k_device") or probe_dict is not None:
+            sim.build_probe(probe_dict) if probe_dict is not None else sim.build_probe()
+        self.norm = float(sim.normalization)
+        self.s = float(Nx * math.sqrt(Ny * Nx / self.norm))
+        self.sv = float(sim.sigma)
+
+    # state helpers (B planes of the row-transformed wave)
+    def init(self, ws, pos):
+        B = ws.shape[0]
+        call("namsa_probe_init", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
+             np.float32(self.sim.kmax), ptr(ws), stream_handle())
+
+    def step(self, ws, t, pm):
+        B, st = ws.shape[0], stream_handle()
+        call("namsa_slice_rows", ptr(ws), B, self.Ny, self.Nx, ptr(t), 0, self.Nx, st)
+        call("namsa_slice_cols", ptr(ws), B, self.Ny, self.Nx, ptr(pm), 0, self.Nx, st)
+
+    def fine_steps(self, ws, k):
+        for j in range(self.factor):
+            self.step(ws, self.t_f[self.factor * k + j], self.pm_f)
+
+    def to_real(self, ws):
+        call("namsa_row_fft", ptr(ws), ws.shape[0], self.Ny, self.Nx, 1, np.float32(1.0 / self.Nx),
+             stream_handle())
+
+    def to_state(self, x):
+        call("namsa_row_fft", ptr(x), x.shape[0], self.Ny, self.Nx, 0, np.float32(1.0), stream_handle())
+
+    def cbed_from_real(self, x, cbed):
+        """CBED of real-space waves x (destroyed): |FFT2 x|^2 Nx^2 / (Ny Nx norm)
+        inside the bandwidth mask (every propagator already zeroes the rest)."""
+        B = x.shape[0]
+        hw = self.Ny * self.Nx
+        call("namsa_fft2d", ptr(x), B, self.Ny, self.Nx, 0, stream_handle())
+        call("namsa_intensity", ptr(x), B * hw, hw, np.float32(self.Nx / (self.Ny * self.norm)),
+             ptr(self.mask), ptr(cbed), stream_handle())
+
+    def n_inputs(self, derivs):
+        return 2 + (6 if derivs else 0) + self.factor
+
+    def pairs(self, pos):
+        """Yield (k, x_coarse, x_fine) for every coarse slice: both real-space
+        waves after coarse slice k, from the same fine-stepped incoming wave."""
+        torch = _torch()
+        B = pos.shape[0]
+        ws = torch.empty((B, self.Ny, self.Nx), dtype=torch.complex64, device=self.sim.device)
+        self.init(ws, pos)
+        for k in range(self.S_c):
+            xc = ws.clone()
+            self.step(xc, self.t_c[k], self.pm_c)
+            self.fine_steps(ws, k)
+            xf = ws.clone()
+            self.to_real(xc)
+            self.to_real(xf)
+            yield k, xc, xf
+
+    def potentials(self, k):
+        return self.V_f[self.factor * k:self.factor * (k + 1)]
+
+
+# ---------------------------------------------------------------------- inference
+def run_plain(mat, pos, coarse=False):
+    """CBEDs of the plain path over the same thickness, fine (factor*S_c
+    slices) or coarse (S_c slices), through the fused batched driver."""
+    torch = _torch()
+    B = pos.shape[0]
+    dev = mat.sim.device
+    ws = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.complex64, device=dev)
+    cbed = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.float32, device=dev)
+    t, S, pm = (mat.t_c, mat.S_c, mat.pm_c) if coarse else (mat.t_f, mat.factor * mat.S_c, mat.pm_f)
+    call("namsa_multislice", ptr(mat.sim.psi_k_device), ptr(pos), B, mat.Ny, mat.Nx, np.float32(mat.sim.kmax),
+         ptr(t), S, ptr(pm), mat.band[0], mat.band[1], B, _lib.OUT_CBED, np.float32(1.0 / mat.norm), ptr(ws),
+         ptr(cbed), None, 0, None, None, stream_handle())
+    return cbed
+
+
+def run_learned(net, mat, pos, tau=None):
+    """CBEDs through S_c coarse steps, each corrected by the network.
+
+    tau: fallback threshold on the predicted relative correction
+    |Delta|^2 / |s x_c|^2 (max over the batch); a coarse slice above it is
+    redone with fine steps.  Returns (cbed, number of fallback slices)."""
+    torch = _torch()
+    if net.cin != mat.n_inputs(net.derivs):
+        raise NamsaError("network input channels do not match the coarsening factor")
+    B = pos.shape[0]
+    hw = mat.Ny * mat.Nx
+    st = stream_handle()
+    ws = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.complex64, device=mat.sim.device)
+    x = torch.empty_like(ws)
+    keep = torch.empty_like(ws) if tau is not None else None
+    net.ensure(B)
+    mat.init(ws, pos)
+    fallbacks = 0
+    for k in range(mat.S_c):
+        if keep is not None:
+            keep.copy_(ws)
+        mat.step(ws, mat.t_c[k], mat.pm_c)
+        x.copy_(ws)
+        mat.to_real(x)
+        call("namsa_corrector_pack", ptr(x), ptr(mat.potentials(k)), mat.factor, 1, int(net.derivs), ptr(net.u),
+             B, mat.Ny, mat.Nx, np.float32(mat.s), np.float32(mat.sv), st)
+        net.forward(B)
+        if keep is not None:
+            call("namsa_corrector_loss", ptr(net.out), ptr(x), ptr(x), B, hw, np.float32(mat.s), ptr(net.sums),
+                 None, st)
+            pred = net.sums[0:2 * B:2].max().item() / hw
+            if pred > tau:
+                ws.copy_(keep)
+                mat.fine_steps(ws, k)
+                fallbacks += 1
+                continue
+        call("namsa_corrector_apply", ptr(net.out), ptr(x), ptr(ws), B, hw, np.float32(mat.s), st)
+        mat.to_state(ws)
+    mat.to_real(ws)
+    cbed = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.float32, device=ws.device)
+    mat.cbed_from_real(ws, cbed)
+    return cbed, fallbacks
+
+
+# ---------------------------------------------------------------------- training
+class PairPool:
+    """Training pairs resident in HBM: coarse and fine real-space waves plus
+    a (material, coarse slice) index for the potentials."""
+
+    def __init__(self):
+        self.xc, self.xf, self.ref = [], [], []
+        self.mats = []
+
+    def add_material(self, mat, pos):
+        """Pairs of every coarse slice for probes at pos, stored at unit RMS
+        (times the material's s) so materials with different probe norms mix."""
+        if self.mats and abs(mat.sv / self.mats[0].sv - 1) > 1e-6:
+            raise NamsaError("materials of one pool must share the beam energy")
+        mi = len(self.mats)
+        self.mats.append(mat)
+        for k, xc, xf in mat.pairs(pos):
+            xc.mul_(mat.s)
+            xf.mul_(mat.s)
+            for b in range(xc.shape[0]):
+                self.xc.append(xc[b])
+                self.xf.append(xf[b])
+                self.ref.append((mi, k))
+
+    def finalize(self):
+        torch = _torch()
+        self.XC = torch.stack(self.xc)
+        self.XF = torch.stack(self.xf)
+        self.xc, self.xf = None, None
+        return self
+
+    def __len__(self):
+        return len(self.ref)
+
+    def batch(self, idx, V_out):
+        """Gather samples idx into contiguous (xc, xf) and V_out [B][factor][H][W]."""
+        torch = _torch()
+        ii = torch.as_tensor(np.asarray(idx), device=self.XC.device)
+        for j, i in enumerate(idx):
+            mi, k = self.ref[i]
+            V_out[j].copy_(self.mats[mi].potentials(k))
+        return self.XC.index_select(0, ii), self.XF.index_select(0, ii)
+
+
+def batch_loss(net, pool, idx, V, grad=False):
+    """Per-sample relative residual |out - Delta|^2 / |Delta|^2 of samples idx
+    (and, with grad, the gradient of its mean in net.grad)."""
+    mat = pool.mats[pool.ref[idx[0]][0]]
+    B = len(idx)
+    hw = mat.Ny * mat.Nx
+    st = stream_handle()
+    xc, xf = pool.batch(idx, V[:B])
+    one = np.float32(1.0)      # pool waves are stored already scaled by s
+    call("namsa_corrector_pack", ptr(xc), ptr(V), mat.factor, 0, int(net.derivs), ptr(net.u), B, mat.Ny,
+         mat.Nx, one, np.float32(mat.sv), st)
+    net.forward(B)
+    call("namsa_corrector_loss", ptr(net.out), ptr(xc), ptr(xf), B, hw, one, ptr(net.sums),
+         ptr(net.gout) if grad else None, st)
+    if grad:
+        net.backward(B)
+    return net.sums[:2 * B].view(B, 2)
+
+
+def train(net, pool, steps, batch=8, lr=2e-3, seed=0, log_every=0, log=print):
+    """Adam with cosine decay on random minibatches of the pool; returns the
+    per-log-step mean relative residuals."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    net.ensure(batch)
+    V = torch.empty((batch, pool.mats[0].factor, net.H, net.W), dtype=torch.float32, device=net.device)
+    hist = []
+    for it in range(steps):
+        idx = rng.choice(len(pool), size=batch, replace=False)
+        sums = batch_loss(net, pool, idx, V, grad=True)
+        net.adam(lr * 0.5 * (1 + math.cos(math.pi * it / steps)))
+        if log_every and (it % log_every == 0 or it == steps - 1):
+            r = float((sums[:, 0] / sums[:, 1]).mean())
+            hist.append((it, r))
+            log(f"step {it:5d}  relative residual {r:.4f}")
+    return hist
+
+
+def evaluate(net, pool, batch=8):
+    """Mean relative residual over the whole pool, corrected and uncorrected (= 1)."""
+    torch = _torch()
+    net.ensure(batch)
+    V = torch.empty((batch, pool.mats[0].factor, net.H, net.W), dtype=torch.float32, device=net.device)
+    num = den = 0.0
+    per = []
+    for i0 in range(0, len(pool), batch):
+        idx = list(range(i0, min(len(pool), i0 + batch)))
+        s = batch_loss(net, pool, idx, V).cpu().numpy()
+        per.extend(list(s[:, 0] / s[:, 1]))
+        num += s[:, 0].sum()
+        den += s[:, 1].sum()
+    return float(num / den), np.asarray(per)

@@ -171,6 +171,14 @@ int namsa_fft2d(void* data, int B, int Ny, int Nx, int inverse, void* stream) {
   return row_pass(Nx, ROW_INV, r, st);
 }
 
+int namsa_row_fft(void* data, int B, int Ny, int Nx, int inverse, float scale, void* stream) {
+  if (int rc = check_grid(Ny, Nx, "namsa_row_fft")) return rc;
+  if (!data || B <= 0) return fail(kErrArg, "namsa_row_fft: null data or B=%d", B);
+  if (!inverse && scale != 1.0f) return fail(kErrArg, "namsa_row_fft: the forward transform is unscaled");
+  const RowArgs r = row_args(static_cast<float2*>(data), B, Ny, Nx, nullptr, scale, 0, Nx);
+  return row_pass(Nx, inverse ? ROW_INV : ROW_FWD, r, as_stream(stream));
+}
+
 int namsa_probe_k(int Ny, int Nx, float kmax, float k_semi, float lambda, float C1, float C3,
                   float C5, int smooth, float steepness, void* psi_k, float* apert,
                   void* stream) {

@@ -97,11 +97,13 @@ struct SiteBox {
   int z, y0, r_lo, r_hi, x0, c_lo, c_hi;
 };
 
+// POT: write the projected potential V (float) instead of t = exp(i sigma V).
+template <bool POT>
 __global__ void __launch_bounds__(kSliceThreads)
 build_slices_kernel(const float* __restrict__ tables, int nZ, int Py, int Px,
                     const int* __restrict__ sites,
                     const int* __restrict__ offsets, int Ny, int Nx, float sigma,
-                    float2* __restrict__ t_out) {
+                    float2* __restrict__ t_out, float* __restrict__ v_out) {
   __shared__ SiteBox list[kSliceThreads];
   __shared__ int wave_count[kSliceThreads / 64];
 
@@ -165,6 +167,10 @@ build_slices_kernel(const float* __restrict__ tables, int nZ, int Py, int Px,
   for (int k = 0; k < kPxPerThread; ++k) {
     const int row = ty0 + wave + 4 * k;
     if (row < Ny && col < Nx) {
+      if constexpr (POT) {
+        v_out[s * plane + static_cast<long long>(row) * Nx + col] = acc[k];
+        continue;
+      }
       float sn, cs;
       sincosf(acc[k] * sigma, &sn, &cs);
       if (isnan(cs)) cs = 1.0f;
@@ -200,8 +206,21 @@ int namsa_build_slices(const float* tables, int nZ, int Py, int Px, const int* s
   if (S == 0) return kOk;
   if (S > 65535) return fail(kErrArg, "namsa_build_slices: S=%d exceeds the grid z limit", S);
   dim3 grid((Nx + kTileW - 1) / kTileW, (Ny + kTileH - 1) / kTileH, S);
-  hipLaunchKernelGGL(build_slices_kernel, grid, dim3(kSliceThreads), 0, as_stream(stream), tables,
-                     nZ, Py, Px, sites, slice_offsets, Ny, Nx, sigma, static_cast<float2*>(t_out));
+  hipLaunchKernelGGL(build_slices_kernel<false>, grid, dim3(kSliceThreads), 0, as_stream(stream), tables,
+                     nZ, Py, Px, sites, slice_offsets, Ny, Nx, sigma, static_cast<float2*>(t_out), nullptr);
+  return check_hip(hipGetLastError(), "build_slices_kernel launch");
+}
+
+int namsa_build_potential(const float* tables, int nZ, int Py, int Px, const int* sites,
+                          const int* slice_offsets, int S, int Ny, int Nx, float* v_out, void* stream) {
+  if (!tables || !slice_offsets || !v_out || nZ <= 0 || Py <= 0 || Px <= 0 || S < 0 || Ny <= 0 ||
+      Nx <= 0)
+    return fail(kErrArg, "namsa_build_potential: bad arguments");
+  if (S == 0) return kOk;
+  if (S > 65535) return fail(kErrArg, "namsa_build_potential: S=%d exceeds the grid z limit", S);
+  dim3 grid((Nx + kTileW - 1) / kTileW, (Ny + kTileH - 1) / kTileH, S);
+  hipLaunchKernelGGL(build_slices_kernel<true>, grid, dim3(kSliceThreads), 0, as_stream(stream), tables,
+                     nZ, Py, Px, sites, slice_offsets, Ny, Nx, 0.0f, nullptr, v_out);
   return check_hip(hipGetLastError(), "build_slices_kernel launch");
 }
 

@@ -1,0 +1,269 @@
+"""GPU kernels of the coarse-slice corrector (MI355X): the BF16 MFMA GEMM
+against float64 numpy products of the same bf16-rounded operands (FP32
+accumulation: tolerance 2e-5 of the result norm), and the pointwise kernels
+against closed forms."""
+import sys
+
+import numpy as np
+import pytest
+
+from namsa_amd import _lib
+from namsa_amd._lib import call, ptr
+from oracle import fno_oracle as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bf16(a):
+    """Round-to-nearest-even float32 -> bfloat16, returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _gemm(torch, A, sA, B, sB, Cbuf, sC, M, N, K, batch=1, split=1, alpha=1.0, beta=0.0, bias=None,
+          bias_mode=0, gelu=False, P=None):
+    call("namsa_gemm_bf16", ptr(A), *sA, ptr(B), *sB, ptr(Cbuf), *sC, M, N, K, batch, split,
+         np.float32(alpha), np.float32(beta), ptr(bias) if bias is not None else None, bias_mode,
+         1 if gelu else 0, ptr(P) if P is not None else None, _lib.stream_handle())
+    torch.cuda.synchronize()
+
+
+def _view(flat, b, s, rows, cols):
+    """Gather the (rows x cols) matrix of batch b from strides s = (sb, si, sj)."""
+    i = np.arange(rows)[:, None]
+    j = np.arange(cols)[None, :]
+    return flat[b * s[0] + i * s[1] + j * s[2]]
+
+
+@pytest.mark.parametrize("M,N,K,batch,a_t,b_t", [
+    (64, 64, 32, 1, False, False),     # one tile, one k step
+    (32, 32, 16, 1, False, False),     # a single MFMA block (checks the lane layout)
+    (100, 77, 45, 3, False, False),    # ragged edges in every dimension, batched
+    (70, 130, 96, 2, True, False),     # transposed A
+    (70, 130, 96, 2, False, True),     # transposed B
+    (33, 65, 200, 4, True, True),
+])
+def test_gemm_layouts(gpu, M, N, K, batch, a_t, b_t):
+    import torch
+    _say(f"gemm M={M} N={N} K={K} batch={batch} aT={a_t} bT={b_t}")
+    rng = np.random.default_rng(M * 7 + N + K)
+    sA = (M * K + 5, 1, M) if a_t else (M * K + 5, K, 1)
+    sB = (K * N + 3, 1, K) if b_t else (K * N + 3, N, 1)
+    sC = (M * N + 1, N, 1)
+    A = rng.standard_normal(batch * sA[0]).astype(np.float32)
+    B = rng.standard_normal(batch * sB[0]).astype(np.float32)
+    Ad, Bd = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    Cd = torch.full((batch * sC[0],), 7.0, dtype=torch.float32, device="cuda")
+    _gemm(torch, Ad, sA, Bd, sB, Cd, sC, M, N, K, batch=batch)
+    C = Cd.cpu().numpy()
+    Ab, Bb = bf16(A), bf16(B)
+    for b in range(batch):
+        ref = _view(Ab, b, sA, M, K) @ _view(Bb, b, sB, K, N)
+        got = _view(C, b, sC, M, N)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert err < 2e-5, (b, err)
+    # the padding word between batches is untouched
+    assert C[M * N] == 7.0
+
+
+def test_gemm_epilogue_bias_beta_gelu_and_split_k(gpu):
+    import torch
+    _say("gemm epilogues")
+    rng = np.random.default_rng(5)
+    M, N, K = 96, 80, 300
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32) * 0.1
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    bias_i = rng.standard_normal(M).astype(np.float32)
+    bias_j = rng.standard_normal(N).astype(np.float32)
+    Ad, Bd = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    ref = bf16(A) @ bf16(B)
+    s = (0, K, 1), (0, N, 1), (0, N, 1)
+    # alpha, beta, row bias, GELU with the pre-activation stored
+    Cd = torch.from_numpy(C0.copy()).cuda()
+    Pd = torch.zeros_like(Cd)
+    _gemm(torch, Ad, s[0], Bd, s[1], Cd, s[2], M, N, K, alpha=0.5, beta=2.0,
+          bias=torch.from_numpy(bias_i).cuda(), bias_mode=1, gelu=True, P=Pd)
+    pre = 0.5 * ref + 2.0 * C0 + bias_i[:, None]
+    np.testing.assert_allclose(Pd.cpu().numpy(), pre, rtol=0, atol=2e-5 * np.abs(pre).max())
+    np.testing.assert_allclose(Cd.cpu().numpy(), F.gelu(pre), rtol=0, atol=2e-5 * np.abs(pre).max())
+    # column bias
+    Cd = torch.zeros((M, N), dtype=torch.float32, device="cuda")
+    _gemm(torch, Ad, s[0], Bd, s[1], Cd, s[2], M, N, K, bias=torch.from_numpy(bias_j).cuda(), bias_mode=2)
+    np.testing.assert_allclose(Cd.cpu().numpy(), ref + bias_j[None], rtol=0, atol=2e-5 * np.abs(ref).max())
+    # split-K adds onto C (atomics): 5 splits of a K=300 product onto C0
+    Cd = torch.from_numpy(C0.copy()).cuda()
+    _gemm(torch, Ad, s[0], Bd, s[1], Cd, s[2], M, N, K, split=5, alpha=-1.0)
+    np.testing.assert_allclose(Cd.cpu().numpy(), C0 - ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+
+
+def test_gemm_rejects_bad_arguments(gpu):
+    import torch
+    x = torch.zeros(16, device="cuda")
+    lib = _lib.load()
+    st = _lib.stream_handle()
+    rc = lib.namsa_gemm_bf16(ptr(x), 0, 1, 1, ptr(x), 0, 1, 1, ptr(x), 0, 1, 1, 4, 4, 4, 1, 2,
+                             1.0, 0.0, ptr(x), 1, 0, None, st)
+    assert rc != 0 and b"split-K" in lib.namsa_last_error()
+    rc = lib.namsa_gemm_bf16(ptr(x), 0, 1, 1, ptr(x), 0, 1, 1, ptr(x), 0, 1, 1, 4, 4, 4, 1, 1,
+                             1.0, 0.0, None, 0, 1, None, st)
+    assert rc != 0
+
+
+def test_pointwise_kernels(gpu):
+    import torch
+    _say("fill / gelu backward / adam / pack / loss / apply / intensity")
+    rng = np.random.default_rng(9)
+    n = 5000
+    x = torch.empty(n, device="cuda")
+    call("namsa_fill", ptr(x), n, np.float32(2.5), _lib.stream_handle())
+    assert torch.all(x == 2.5)
+    pre = rng.standard_normal(n).astype(np.float32) * 3
+    g = rng.standard_normal(n).astype(np.float32)
+    gd = torch.from_numpy(g.copy()).cuda()
+    call("namsa_gelu_backward", ptr(gd), ptr(torch.from_numpy(pre).cuda()), n, _lib.stream_handle())
+    # fp32 tanh: absolute error ~1e-6 where gelu' passes through zero near x = -0.75 and x < -3
+    np.testing.assert_allclose(gd.cpu().numpy(), g * F.gelu_grad(pre.astype(np.float64)), rtol=2e-5, atol=1e-5)
+    # Adam, two steps, against the textbook update with decoupled decay
+    p0 = rng.standard_normal(n).astype(np.float32)
+    grads = [rng.standard_normal(n).astype(np.float32) for _ in range(2)]
+    pd, md, vd = torch.from_numpy(p0.copy()).cuda(), torch.zeros(n, device="cuda"), torch.zeros(n, device="cuda")
+    p, m, v = p0.astype(np.float64), np.zeros(n), np.zeros(n)
+    lr, b1, b2, eps, wd = 1e-2, 0.9, 0.999, 1e-8, 1e-3
+    for step, gr in enumerate(grads, 1):
+        call("namsa_adam", ptr(pd), ptr(torch.from_numpy(gr).cuda()), ptr(md), ptr(vd), n, np.float32(lr),
+             np.float32(b1), np.float32(b2), np.float32(eps), step, np.float32(wd), _lib.stream_handle())
+        m = b1 * m + (1 - b1) * gr
+        v = b2 * v + (1 - b2) * gr.astype(np.float64) ** 2
+        p = p - lr * ((m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps) + wd * p)
+    np.testing.assert_allclose(pd.cpu().numpy(), p, rtol=1e-5, atol=1e-6)
+    # pack / loss / apply on B = 3 samples of hw = 40 with nv = 2 potential channels
+    B, hw, nv, s, sv = 3, 40, 2, 4.0, 0.5
+    pc = (rng.standard_normal((B, hw)) + 1j * rng.standard_normal((B, hw))).astype(np.complex64)
+    pf = (pc + 0.1 * (rng.standard_normal((B, hw)) + 1j * rng.standard_normal((B, hw)))).astype(np.complex64)
+    V = rng.standard_normal((B, nv, hw)).astype(np.float32)
+    pcd, pfd = torch.from_numpy(pc).cuda(), torch.from_numpy(pf).cuda()
+    u = torch.empty((2 + nv, B * hw), device="cuda")
+    Ny, Nx = 5, 8                                   # hw = 40
+    call("namsa_corrector_pack", ptr(pcd), ptr(torch.from_numpy(V).cuda()), nv, 0, 0, ptr(u), B, Ny, Nx,
+         np.float32(s), np.float32(sv), _lib.stream_handle())
+    un = u.cpu().numpy()
+    np.testing.assert_allclose(un[0], pc.real.ravel() * s, rtol=1e-6)
+    np.testing.assert_allclose(un[1], pc.imag.ravel() * s, rtol=1e-6)
+    np.testing.assert_allclose(un[2:], V.transpose(1, 0, 2).reshape(nv, -1) * sv, rtol=1e-6)
+    call("namsa_corrector_pack", ptr(pcd), ptr(torch.from_numpy(V[0].copy()).cuda()), nv, 1, 0, ptr(u), B, Ny,
+         Nx, np.float32(s), np.float32(sv), _lib.stream_handle())
+    np.testing.assert_allclose(u.cpu().numpy()[2:], np.tile(V[0], (1, B)) * sv, rtol=1e-6)
+    # with derivative channels: periodic Laplacian and central differences per sample plane
+    u8 = torch.empty((8 + nv, B * hw), device="cuda")
+    call("namsa_corrector_pack", ptr(pcd), ptr(torch.from_numpy(V).cuda()), nv, 0, 1, ptr(u8), B, Ny, Nx,
+         np.float32(s), np.float32(sv), _lib.stream_handle())
+    w = pc.astype(np.complex128).reshape(B, Ny, Nx) * s
+    r = lambda a, d, ax: np.roll(a, d, axis=ax)
+    lap = r(w, 1, 1) + r(w, -1, 1) + r(w, 1, 2) + r(w, -1, 2) - 4 * w
+    dy = 0.5 * (r(w, -1, 1) - r(w, 1, 1))
+    dx = 0.5 * (r(w, -1, 2) - r(w, 1, 2))
+    got = u8.cpu().numpy()
+    for c, f in enumerate((w, lap, dy, dx)):
+        np.testing.assert_allclose(got[2 * c], f.real.ravel(), rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(got[2 * c + 1], f.imag.ravel(), rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(got[8:], V.transpose(1, 0, 2).reshape(nv, -1) * sv, rtol=1e-6)
+    out = rng.standard_normal((2, B * hw)).astype(np.float32)
+    outd = torch.from_numpy(out).cuda()
+    sums = torch.empty(2 * B, dtype=torch.float64, device="cuda")
+    gout = torch.empty((2, B * hw), device="cuda")
+    call("namsa_corrector_loss", ptr(outd), ptr(pcd), ptr(pfd), B, hw, np.float32(s), ptr(sums), ptr(gout),
+         _lib.stream_handle())
+    d = ((pf.astype(np.complex128) - pc) * s).reshape(-1)
+    o = out[0].astype(np.float64) + 1j * out[1]
+    r = (o - d).reshape(B, hw)
+    num = np.sum(np.abs(r) ** 2, axis=1)
+    den = np.sum(np.abs(d.reshape(B, hw)) ** 2, axis=1)
+    np.testing.assert_allclose(sums.cpu().numpy().reshape(B, 2), np.stack([num, den], 1), rtol=1e-5)
+    gref = (r * (2 / (den[:, None] * B))).reshape(-1)
+    np.testing.assert_allclose(gout.cpu().numpy(), np.stack([gref.real, gref.imag]), rtol=1e-4, atol=1e-7)
+    psi = torch.empty_like(pcd)
+    call("namsa_corrector_apply", ptr(outd), ptr(pcd), ptr(psi), B, hw, np.float32(s), _lib.stream_handle())
+    np.testing.assert_allclose(psi.cpu().numpy().ravel(), pc.ravel() + o / s, rtol=1e-5, atol=1e-6)
+    I = torch.empty(B * hw, device="cuda")
+    call("namsa_intensity", ptr(pcd), B * hw, hw, np.float32(0.25), None, ptr(I), _lib.stream_handle())
+    np.testing.assert_allclose(I.cpu().numpy(), np.abs(pc.ravel()) ** 2 * 0.25, rtol=1e-5)
+    mask = (rng.random(hw) > 0.5).astype(np.float32)
+    call("namsa_intensity", ptr(pcd), B * hw, hw, np.float32(0.25), ptr(torch.from_numpy(mask).cuda()), ptr(I),
+         _lib.stream_handle())
+    np.testing.assert_allclose(I.cpu().numpy(), np.abs(pc.ravel()) ** 2 * 0.25 * np.tile(mask, B), rtol=1e-5)
+
+
+def test_mode_bookkeeping(gpu):
+    import torch
+    _say("modes combine / expand")
+    rng = np.random.default_rng(4)
+    C, B, m = 3, 2, 4
+    m2 = 2 * m
+    P = rng.standard_normal((C * B, m2, m2)).astype(np.float32)
+    Xs = torch.empty((m * m, 2 * C, B), device="cuda")
+    call("namsa_modes_combine", ptr(torch.from_numpy(P).cuda()), ptr(Xs), C, B, m, _lib.stream_handle())
+    got = Xs.cpu().numpy()
+    Pc = P.reshape(C, B, 2, m, 2, m)                     # [c][b][r][p][t][q]
+    Xr = Pc[:, :, 0, :, 0, :] + Pc[:, :, 1, :, 1, :]      # [c][b][p][q]
+    Xi = Pc[:, :, 0, :, 1, :] - Pc[:, :, 1, :, 0, :]
+    ref = np.concatenate([Xr, Xi], 0).transpose(2, 3, 0, 1).reshape(m * m, 2 * C, B)
+    np.testing.assert_allclose(got, ref, rtol=1e-6)
+    Ys = rng.standard_normal((m * m, 2 * C, B)).astype(np.float32)
+    E = torch.empty((C * B, m2, m2), device="cuda")
+    call("namsa_modes_expand", ptr(torch.from_numpy(Ys).cuda()), ptr(E), C, B, m, _lib.stream_handle())
+    e = E.cpu().numpy().reshape(C, B, 2, m, 2, m)
+    Y = Ys.reshape(m, m, 2, C, B).transpose(2, 3, 4, 0, 1)   # [r][o][b][p][q]
+    np.testing.assert_array_equal(e[:, :, 0, :, 0, :], Y[0])
+    np.testing.assert_array_equal(e[:, :, 1, :, 0, :], -Y[1])
+    np.testing.assert_array_equal(e[:, :, 0, :, 1, :], Y[1])
+    np.testing.assert_array_equal(e[:, :, 1, :, 1, :], Y[0])
+
+
+def test_row_fft_round_trip_and_exit_wave(gpu):
+    """Inverse row transform (scale 1/Nx) of the slice state is the exit wave
+    the EXIT mode writes (times its out_scale), and the forward transform
+    takes it back to the state."""
+    import torch
+    _say("row transforms vs EXIT mode")
+    from namsa_amd.msa import MSAGPU
+    from tests.parity_util import random_block
+    rec = random_block(3, 24.0, 10.0, 300)
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    sim.build_probe()
+    sim.probe_positions = np.array([[0.5, -1.0], [2.0, 3.0]])
+    sim.num_probes = 2
+    sim.plan_simulation()
+    sim.multislice(output="exit_wave")
+    exit_ref = sim.probes_device.clone()                     # scaled by 1/sqrt(norm)
+    Ny, Nx = sim._grid
+    S = int(sim.num_slices)
+    pos = torch.from_numpy(sim.probe_positions.astype(np.float32)).cuda()
+    ws = torch.empty((2, Ny, Nx), dtype=torch.complex64, device="cuda")
+    st = _lib.stream_handle()
+    call("namsa_probe_init", ptr(sim.psi_k_device), ptr(pos), 2, Ny, Nx, np.float32(sim.kmax), ptr(ws), st)
+    for i in range(S):
+        call("namsa_slice_rows", ptr(ws), 2, Ny, Nx, ptr(sim.potential_slices_device[i]), 0, Nx, st)
+        call("namsa_slice_cols", ptr(ws), 2, Ny, Nx, ptr(sim.pm_device), 0, Nx, st)
+    state = ws.clone()
+    call("namsa_row_fft", ptr(ws), 2, Ny, Nx, 1, np.float32(1.0 / Nx), st)
+    torch.cuda.synchronize()
+    x = ws.cpu().numpy().astype(np.complex128)
+    ref = exit_ref.cpu().numpy() * np.sqrt(sim.normalization) / Nx
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-5
+    call("namsa_row_fft", ptr(ws), 2, Ny, Nx, 0, np.float32(1.0), st)
+    torch.cuda.synchronize()
+    back = ws.cpu().numpy()
+    s0 = state.cpu().numpy()
+    assert np.linalg.norm(back - s0) / np.linalg.norm(s0) < 1e-5
+    lib = _lib.load()
+    assert lib.namsa_row_fft(ptr(ws), 2, Ny, Nx, 0, 0.5, st) != 0
