@@ -362,7 +362,8 @@ def main():
     frac = (hi - lo) / Nx
     strip = 16 if "c16" in sched["col"] else (8 if "c8" in sched["col"] else 4)
     strips = (-(-hi // strip) - lo // strip) * strip / Nx
-    row_bytes = int((8 * frac + 8) * plane * G + 8 * plane)  # band read + full write + slice
+    sfrac = (min(Nx, -(-hi // 16) * 16) - lo // 16 * 16) / Nx
+    row_bytes = int((8 * frac + 8 * sfrac) * plane * G + 8 * plane)  # band read + aligned-band write + slice
     col_bytes = int(16 * strips * plane * G)                 # band strips read + write
     nominal = 32 * plane * G + 8 * plane                     # two full round trips per slice
     kernels = {"row_pass": (row_ms, row_bytes), "col_pass": (col_ms, col_bytes)}

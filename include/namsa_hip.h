@@ -159,6 +159,9 @@ size_t namsa_det_workspace_doubles(int B, int Ny, int Nx, int nDet);
  * A slice i is namsa_slice_rows(t + i*Ny*Nx) followed by namsa_slice_cols.
  * band_lo, band_hi: the columns of pm that may be nonzero (see below); the
  * row pass right after namsa_probe_init must use the full band [0, Nx).
+ * namsa_slice_rows reads only the band and stores only the band widened to
+ * multiples of 16 columns, the only columns the namsa_slice_cols with the
+ * same band reads (with the full band it stores every column).
  */
 int namsa_probe_init(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
                      void* psi_ws, void* stream);

@@ -134,6 +134,8 @@ RowArgs row_args(float2* psi, int B, int Ny, int Nx, const float2* tsl, float sc
   r.scale = scale;
   r.band_lo = lo;
   r.band_hi = hi;
+  r.store_lo = 0;
+  r.store_hi = Nx;
   return r;
 }
 
@@ -236,10 +238,11 @@ int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, i
   if (int rc = check_grid(Ny, Nx, "namsa_slice_rows")) return rc;
   if (B <= 0 || !psi_ws || !t_slice) return fail(kErrArg, "namsa_slice_rows: bad arguments");
   if (int rc = check_band(Nx, band_lo, band_hi, "namsa_slice_rows")) return rc;
-  return row_pass(Nx, ROW_STEP,
-                  row_args(static_cast<float2*>(psi_ws), B, Ny, Nx,
-                           static_cast<const float2*>(t_slice), 1.0f, band_lo, band_hi),
-                  as_stream(stream));
+  RowArgs r = row_args(static_cast<float2*>(psi_ws), B, Ny, Nx, static_cast<const float2*>(t_slice), 1.0f,
+                       band_lo, band_hi);
+  r.store_lo = band_lo / kStoreAlign * kStoreAlign;
+  r.store_hi = std::min(Nx, (band_hi + kStoreAlign - 1) / kStoreAlign * kStoreAlign);
+  return row_pass(Nx, ROW_STEP, r, as_stream(stream));
 }
 
 int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
@@ -301,11 +304,19 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
     }
     // The probe former's spectrum fills every column; after the first
     // propagator only the band can be nonzero.
+    // Each row step's output is read only by the column pass that follows
+    // (or the readout), which touches only the column strips meeting the
+    // band; strips are at most kStoreAlign wide and aligned to their width,
+    // so the row pass stores just the kStoreAlign-aligned band.
     const int last_prop = (mode == NAMSA_OUT_EXIT) ? S : S - 1;
+    const int store_lo = band_lo / kStoreAlign * kStoreAlign;
+    const int store_hi = std::min(Nx, (band_hi + kStoreAlign - 1) / kStoreAlign * kStoreAlign);
     for (int i = 0; i < S; ++i) {
       const int lo = i == 0 ? 0 : band_lo, hi = i == 0 ? Nx : band_hi;
-      if (int rc = row_pass(Nx, ROW_STEP, row_args(psi, Bg, Ny, Nx, tt + i * plane, 1.0f, lo, hi), st))
-        return rc;
+      RowArgs r = row_args(psi, Bg, Ny, Nx, tt + i * plane, 1.0f, lo, hi);
+      r.store_lo = store_lo;
+      r.store_hi = store_hi;
+      if (int rc = row_pass(Nx, ROW_STEP, r, st)) return rc;
       if (i < last_prop)
         if (int rc = col_pass(Ny, COL_PROP, a, st)) return rc;
     }

@@ -16,7 +16,8 @@ struct RowArgs {
   int rows_per_image;
   const float2* tsl;  // STEP: transmission slice
   float scale;        // INV: output scale
-  int band_lo, band_hi;
+  int band_lo, band_hi;    // columns loaded (the rest are zero on input)
+  int store_lo, store_hi;  // columns stored (the rest are never read afterwards)
 };
 
 // Column pass over columns of length N = Ny of B images [B][Ny][Nx].  The
@@ -46,6 +47,9 @@ struct ColArgs {
 // Narrowest column tile of any compiled schedule: detector scratch is sized
 // for it so that any schedule choice fits.
 constexpr int kMinColTile = 4;
+constexpr int kMaxColTile = 16;
+// row steps followed by a band-limited column pass store this-aligned bands
+constexpr int kStoreAlign = kMaxColTile;
 
 bool supported(int n);
 int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st);

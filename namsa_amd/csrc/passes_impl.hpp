@@ -53,6 +53,7 @@ struct RowS {
 
 template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false>
 struct ColS {
+  static_assert(C_ >= kMinColTile && C_ <= kMaxColTile, "column tile width out of range");
   static constexpr int N = N_, E = E_, C = C_;
   using RL = RL_;
   static constexpr bool PF = PF_, NT = NT_, DB = DB_;
@@ -137,7 +138,10 @@ __global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
     if (row < a.n_rows) {
       float2* p = a.psi + row * N + t;
 #pragma unroll
-      for (int m = 0; m < E; ++m) st<S::NT>(p + T * m, v[m]);
+      for (int m = 0; m < E; ++m) {
+        const int x = t + T * m;
+        if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + T * m, v[m]);
+      }
     }
   }
 }
