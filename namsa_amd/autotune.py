@@ -59,22 +59,33 @@ def _device_key() -> str:
 
 
 def _time(fn, reps: int) -> float:
+    """Median over 3 trials of the mean time of `reps` back-to-back calls (ms),
+    bracketed by one pair of events (no gaps between the launches)."""
     import torch
 
     fn()
     fn()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
-    ev[0].record()
-    for i in range(reps):
-        fn()
-        ev[i + 1].record()
-    torch.cuda.synchronize()
-    return float(np.median([ev[i].elapsed_time(ev[i + 1]) for i in range(reps)]))
+    trials = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        trials.append(e0.elapsed_time(e1) / reps)
+    return float(np.median(trials))
 
 
-def tune(n: int, batch: int = 0, reps: int = 5, write_cache: bool = True, verbose: bool = False,
+def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbose: bool = False,
          band=None) -> dict:
-    """Time every schedule of both passes at n x n with `batch` waves; select the fastest.
+    """Select the fastest schedule of each pass at n x n with `batch` waves.
+
+    Passes are timed the way the slice loop runs them: `reps` slice steps
+    (row pass, then column pass) back to back.  Column variants are swept
+    with the current row variant, then row variants with the best column one
+    (the two passes share no state but the wave).  Returned times are per
+    slice step.
 
     batch: waves per launch (default: the cache-sized probe group production uses).
     band: the propagator's open column range (default: the 1/3-bandwidth band).
@@ -87,29 +98,29 @@ def tune(n: int, batch: int = 0, reps: int = 5, write_cache: bool = True, verbos
     dev = torch.device("cuda", torch.cuda.current_device())
     rng = np.random.default_rng(0)
     psi = torch.randn((batch, n, n), dtype=torch.complex64, device=dev)
-    ph = torch.from_numpy(rng.uniform(0, 2 * np.pi, (n, n)).astype(np.float32)).to(dev)
+    ph = torch.from_numpy(rng.uniform(0, 2 * np.pi, (4, n, n)).astype(np.float32)).to(dev)
     t = torch.polar(torch.ones_like(ph), ph).to(torch.complex64)
-    pm = torch.polar(torch.ones_like(ph), ph.flip(0)).to(torch.complex64)
+    pm = t[0].flip(0).contiguous()
     st = stream_handle()
     lo, hi = band if band is not None else default_band(n)
     result = {"n": n, "batch": batch, "band": [lo, hi], "row": {}, "col": {}}
-    for pass_, key in ((PASS_ROW, "row"), (PASS_COL, "col")):
-        keep = selected(n, pass_)
+    i = [0]
+
+    def step():
+        call("namsa_slice_rows", ptr(psi), batch, n, n, ptr(t[1 + i[0] % 3]), lo, hi, st)
+        call("namsa_slice_cols", ptr(psi), batch, n, n, ptr(pm), lo, hi, st)
+        i[0] += 1
+
+    for pass_, key in ((PASS_COL, "col"), (PASS_ROW, "row")):
         for v, name in enumerate(schedules(n, pass_)):
             select(n, pass_, v)
-            if pass_ == PASS_ROW:
-                fn = lambda: call("namsa_slice_rows", ptr(psi), batch, n, n, ptr(t), lo, hi, st)
-            else:
-                fn = lambda: call("namsa_slice_cols", ptr(psi), batch, n, n, ptr(pm), lo, hi, st)
-            result[key][name] = _time(fn, reps)
+            result[key][name] = _time(step, reps)
             if verbose:
-                print(f"  n={n} {key} {name}: {result[key][name]:.3f} ms", flush=True)
-        select(n, pass_, keep)
-    best_row = min(result["row"], key=result["row"].get)
-    best_col = min(result["col"], key=result["col"].get)
-    select(n, PASS_ROW, best_row)
-    select(n, PASS_COL, best_col)
-    result["best"] = {"row": best_row, "col": best_col}
+                print(f"  n={n} {key} {name}: {result[key][name]:.4f} ms per slice step", flush=True)
+        best = min(result[key], key=result[key].get)
+        select(n, pass_, best)
+    result["best"] = {"row": min(result["row"], key=result["row"].get),
+                      "col": min(result["col"], key=result["col"].get)}
     if write_cache:
         _store(n, result["best"])
     return result

@@ -62,6 +62,10 @@ using C5 = ColS<512, 8, Radices<8, 8, 8>, 8, true, true>;
 using C6 = ColS<512, 16, Radices<16, 16, 2>, 16, false, false>;
 using C7 = ColS<512, 8, Radices<8, 8, 8>, 16, false, false, true>;
 using C8 = ColS<512, 16, Radices<8, 8, 8>, 16, false, false, true>;
+using C9 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, true>;
+using C10 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false>;
+using C11 = ColS<512, 16, Radices<8, 8, 8>, 16, true, false, true>;
+using C12 = ColS<512, 16, Radices<16, 16, 2>, 16, true, false>;
 static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
                                       {"e8.r888.pf", launch_row<R1, R0>},
                                       {"e8.r888.pf.nt", launch_row<R2, R0>},
@@ -75,7 +79,11 @@ static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e8.r888.c8.pf.nt", launch_col<C5, C0>, 8},
                                       {"e16.r16.16.2.c16", launch_col<C6, C0>, 16},
                                       {"e8.r888.c16.db", launch_col<C7, C0>, 16},
-                                      {"e16.r888.c16.db", launch_col<C8, C0>, 16}};
+                                      {"e16.r888.c16.db", launch_col<C8, C0>, 16},
+                                      {"e8.r888.c16.db.pf", launch_col<C9, C0>, 16},
+                                      {"e8.r888.c16.pf", launch_col<C10, C0>, 16},
+                                      {"e16.r888.c16.db.pf", launch_col<C11, C0>, 16},
+                                      {"e16.r16.16.2.c16.pf", launch_col<C12, C0>, 16}};
 #elif NAMSA_PASS_N == 768
 using R0 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false>;
 using R1 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, true, false>;
