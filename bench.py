@@ -46,9 +46,13 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 1 GiB device copy first (HBM counter calibration)")
     p.add_argument("--kernel-reps", type=int, default=40)
-    p.add_argument("--workload", choices=["config2", "config3", "config4"], default="config2",
+    p.add_argument("--workload", choices=["config2", "config3", "config4", "config5"], default="config2",
                    help="config2: 4D-STEM at 512^2 (headline); config3: 48-material uneven batch; "
-                        "config4: learned coarse-slice corrector vs plain at 256^2")
+                        "config4: learned coarse-slice corrector vs plain at 256^2; config5: 2048^2 "
+                        "polycrystal, 1000 slices, positions streamed in chunks")
+    p.add_argument("--c5-probes", type=int, default=256,
+                   help="config5: raster points timed (evenly spaced over the 64x64 raster; 4096 = all)")
+    p.add_argument("--c5-chunk", type=int, default=128, help="config5: positions per streamed chunk")
     p.add_argument("--train-materials", type=int, default=24, help="config4: training structures")
     p.add_argument("--train-steps", type=int, default=3000, help="config4: Adam steps (batch 8)")
     p.add_argument("--streams", type=int, default=4, help="config3: streams the batch is dealt over")
@@ -77,6 +81,66 @@ def setup(edge):
     dets = np.stack([sim.detector_mask(0.0, 25e-3), sim.detector_mask(40e-3, 200e-3)])
     torch.cuda.synchronize()
     return sim, dets
+
+
+def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
+    """Live per-kernel timing of the two slice-step passes on one group of G
+    probes, on the stream the kernels run on: R back-to-back launches of each
+    pass between two HIP events (the same launches namsa_multislice issues; a
+    pass applied twice in a row costs the same as in alternation).  Returns
+    the roofline of the slower (dominant) pass."""
+    import torch
+
+    import namsa_amd._lib as L
+
+    Ny, Nx = sim._grid
+    plane = Ny * Nx
+    S = int(sim.num_slices)
+    st = L.stream_handle()
+    kmax = np.float32(sim.kmax)
+    L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), G, Ny, Nx, kmax, L.ptr(ws), st)
+    L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[0]), 0, Nx, st)
+    L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
+    reps = max(2, reps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    for r in range(reps):
+        L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[1 + r % max(1, S - 1)]), lo, hi, st)
+    ev[1].record()
+    ev[2].record()
+    for r in range(reps):
+        L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
+    ev[3].record()
+    torch.cuda.synchronize()
+    row_ms = ev[0].elapsed_time(ev[1]) / reps
+    col_ms = ev[2].elapsed_time(ev[3]) / reps
+    frac = (hi - lo) / Nx
+    strip = 16 if "c16" in col_sched else (8 if "c8" in col_sched else 4)
+    strips = (-(-hi // strip) - lo // strip) * strip / Nx
+    # Algorithmic bytes per probe-slice: 32 N^2 + 8 N^2 / B (two read+write
+    # round trips of the complex64 wave, the slice read once per launch):
+    # 16 N^2 + 8 N^2 / B for the row pass, 16 N^2 for the column pass.
+    row_alg = 16 * plane * G + 8 * plane
+    col_alg = 16 * plane * G
+    # Bytes the band-limited kernels actually move (loads of the open band,
+    # stores of the 16-aligned band / band strips).
+    sfrac = (min(Nx, -(-hi // 16) * 16) - lo // 16 * 16) / Nx
+    row_band = int((8 * frac + 8 * sfrac) * plane * G + 8 * plane)
+    col_band = int(16 * strips * plane * G)
+    kernels = {"row_pass": (row_ms, row_alg, row_band), "col_pass": (col_ms, col_alg, col_band)}
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    d_ms, d_bytes, d_band = kernels[dom]
+    achieved = d_bytes / (d_ms * 1e-3) / 1e9
+    step_gbs = (row_alg + col_alg) / ((row_ms + col_ms) * 1e-3) / 1e9
+    step_band = (row_band + col_band) / ((row_ms + col_ms) * 1e-3) / 1e9
+    log(f"group {G}: row pass {row_ms * 1e3:.1f} us ({row_alg / row_ms / 1e6:.0f} GB/s algorithmic, "
+        f"{row_band / row_ms / 1e6:.0f} GB/s moved), col pass {col_ms * 1e3:.1f} us ({col_alg / col_ms / 1e6:.0f} "
+        f"GB/s algorithmic, {col_band / col_ms / 1e6:.0f} GB/s moved), slice step {step_gbs:.0f} / {step_band:.0f} GB/s")
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": d_bytes, "avg_launch_ms": d_ms,
+            "launch_probes": G, "band_bytes_per_launch": d_band, "band_GBs": d_band / (d_ms * 1e-3) / 1e9,
+            "band_frac": d_band / (d_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "slice_step_GBs": step_gbs,
+            "slice_step_band_GBs": step_band, "row_ms": row_ms, "col_ms": col_ms}
 
 
 def cpu_baseline(sim, n_probes):
@@ -249,6 +313,91 @@ def bench_config4(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_config5(args):
+    """Seeded 4-grain polycrystal 128 x 128 x 1000 A at 2048^2, dz = 1 A (1000
+    slices), 300 kV, 30 mrad; raster positions streamed through the GPU in
+    chunks with one reused CBED buffer, BF / ADF / total-intensity signals
+    kept for every position."""
+    import torch
+
+    import namsa_amd._lib as L
+    from namsa_amd import DEFAULT_PROBE, MSAGPU, structures
+    from namsa_amd import autotune
+    from namsa_amd.msa import cache_group
+
+    t0 = time.perf_counter()
+    rec, info = structures.polycrystal(seed=5)
+    log(f"config5: polycrystal {info['atoms']} atoms, grains {info['grains']} ({time.perf_counter() - t0:.1f} s)")
+    sim = MSAGPU(300.0, 0.030, rec, sampling=np.array([2048, 2048]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(1.0)
+    sim.build_probe(dict(DEFAULT_PROBE))
+    sim.generate_probe_positions(grid_steps=np.array([64, 64]))
+    sim.build_propagator(1 / 3)
+    torch.cuda.synchronize()
+    log(f"config5: {int(sim.num_slices)} slices at 2048^2 built ({time.perf_counter() - t0:.1f} s)")
+    Ny, Nx = sim._grid
+    S = int(sim.num_slices)
+    dev = sim.device
+    P = max(1, min(args.c5_probes, len(sim.probe_positions)))
+    idx = np.linspace(0, len(sim.probe_positions) - 1, P).astype(int)
+    pos = torch.from_numpy(np.ascontiguousarray(sim.probe_positions[idx], dtype=np.float32)).to(dev)
+    chunk = max(1, min(args.c5_chunk, P))
+    lo, hi = sim.band
+    group = min(cache_group(Ny, Nx), chunk)
+    dets = np.stack([sim.detector_mask(0.0, 30e-3), sim.detector_mask(40e-3, 200e-3),
+                     np.ones((Ny, Nx), np.float32)])
+    det = torch.from_numpy(dets).to(dev)
+    lib = L.load()
+    det_ws = torch.empty(int(lib.namsa_det_workspace_doubles(chunk, Ny, Nx, 3)), dtype=torch.float64, device=dev)
+    det_out = torch.empty((P, 3), dtype=torch.float64, device=dev)
+    cbed = torch.empty((chunk, Ny, Nx), dtype=torch.float32, device=dev)
+    ws = torch.empty((group, Ny, Nx), dtype=torch.complex64, device=dev)
+    scale = np.float32(1.0 / sim.normalization)
+    kmax = np.float32(sim.kmax)
+    st = L.stream_handle()
+
+    def step():
+        for c0 in range(0, P, chunk):
+            B = min(chunk, P - c0)
+            L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos[c0]), B, Ny, Nx, kmax, L.ptr(sim._t), S,
+                   L.ptr(sim.pm_device), lo, hi, group, L.OUT_CBED, scale, L.ptr(ws), L.ptr(cbed), L.ptr(det),
+                   3, L.ptr(det_ws), L.ptr(det_out[c0]), st)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    value = P * S * args.steps / wall
+    sig = det_out.cpu().numpy()
+    log(f"config5: {P} positions x {S} slices in chunks of {chunk}, groups of {group}: {wall / args.steps:.2f} s "
+        f"per step, {value:.4g} probe-slices/s; total intensity {sig[:, 2].min():.4f}..{sig[:, 2].max():.4f}")
+    rf = pass_roofline(sim, pos, ws, group, lo, hi, autotune.schedules(Nx, 1)[autotune.selected(Nx, 1)],
+                       args.kernel_reps)
+    sched = {"row": autotune.schedules(Nx, 0)[autotune.selected(Nx, 0)],
+             "col": autotune.schedules(Nx, 1)[autotune.selected(Nx, 1)], "group": group, "band": [lo, hi]}
+    out = {
+        "metric": METRIC, "value": value, "unit": "probe-positions·slices/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded polycrystal)",
+        "config": {"workload": "2048^2 plain multislice: seeded 4-grain Voronoi polycrystal (Si/GaAs/Au-like "
+                               "lattices, random 3D orientations) 128x128x1000 A, dz = 1 A (1000 slices), 300 kV, "
+                               f"30 mrad, {P} of the 64x64 raster positions streamed in chunks of {chunk}, "
+                               "BF / ADF / total-intensity signals",
+                   "atoms": info["atoms"], "grid": [Ny, Nx], "slices": S, "probe_positions": P, "chunk": chunk,
+                   "method": "plain", "schedules": sched,
+                   "total_intensity_range": [float(sig[:, 2].min()), float(sig[:, 2].max())]},
+        "roofline": rf, "cpu_baseline": None, "finite_outputs": bool(np.isfinite(sig).all()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -264,6 +413,8 @@ def main():
         return bench_config3(args)
     if args.workload == "config4":
         return bench_config4(args)
+    if args.workload == "config5":
+        return bench_config5(args)
 
     copy_gbs = None
     if args.copy_calibration:
@@ -338,50 +489,9 @@ def main():
     log(f"wall {wall:.3f} s, events {gpu_ms / 1e3:.3f} s, {value:.4g} probe-slices/s")
     sane = np.isfinite(det_out.cpu().numpy()).all()
 
-    # Live per-kernel timing on the stream the kernels run on: one group, R
-    # back-to-back launches of each pass between two HIP events (the same
-    # launches the timed region issues inside namsa_multislice; a pass applied
-    # twice in a row costs the same as in alternation).
     G = min(group, P)
-    L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), G, Ny, Nx, kmax, L.ptr(ws), st)
-    L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[0]), 0, Nx, st)
-    L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
-    reps = max(2, args.kernel_reps)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record()
-    for r in range(reps):
-        L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[1 + r % (S - 1)]), lo, hi, st)
-    ev[1].record()
-    ev[2].record()
-    for r in range(reps):
-        L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
-    ev[3].record()
-    torch.cuda.synchronize()
-    row_ms = ev[0].elapsed_time(ev[1]) / reps
-    col_ms = ev[2].elapsed_time(ev[3]) / reps
-    frac = (hi - lo) / Nx
-    strip = 16 if "c16" in sched["col"] else (8 if "c8" in sched["col"] else 4)
-    strips = (-(-hi // strip) - lo // strip) * strip / Nx
-    # Algorithmic bytes per probe-slice: 32 N^2 + 8 N^2 / B (two read+write
-    # round trips of the complex64 wave, the slice read once per launch):
-    # 16 N^2 + 8 N^2 / B for the row pass, 16 N^2 for the column pass.
-    row_alg = 16 * plane * G + 8 * plane
-    col_alg = 16 * plane * G
-    # Bytes the band-limited kernels actually move (loads of the open band,
-    # stores of the 16-aligned band / band strips).
-    sfrac = (min(Nx, -(-hi // 16) * 16) - lo // 16 * 16) / Nx
-    row_band = int((8 * frac + 8 * sfrac) * plane * G + 8 * plane)
-    col_band = int(16 * strips * plane * G)
-    kernels = {"row_pass": (row_ms, row_alg, row_band), "col_pass": (col_ms, col_alg, col_band)}
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    d_ms, d_bytes, d_band = kernels[dom]
-    achieved = d_bytes / (d_ms * 1e-3) / 1e9
-    step_gbs = (row_alg + col_alg) / ((row_ms + col_ms) * 1e-3) / 1e9
-    step_band = (row_band + col_band) / ((row_ms + col_ms) * 1e-3) / 1e9
-    log(f"group {G}: row pass {row_ms * 1e3:.1f} us ({row_alg / row_ms / 1e6:.0f} GB/s algorithmic, "
-        f"{row_band / row_ms / 1e6:.0f} GB/s moved), col pass {col_ms * 1e3:.1f} us ({col_alg / col_ms / 1e6:.0f} "
-        f"GB/s algorithmic, {col_band / col_ms / 1e6:.0f} GB/s moved), slice step {step_gbs:.0f} / {step_band:.0f} GB/s")
-
+    rf = pass_roofline(sim, pos, ws, G, lo, hi, sched["col"], args.kernel_reps)
+    dom = rf["kernel"]
     traffic = None
     tf = ROOT / "profiles" / "hbm_traffic_config2_512.json"
     if tf.exists():
@@ -411,13 +521,7 @@ def main():
                                "(150 slices), 200 kV, 25 mrad, 64x64 raster, CBED + BF/ADF",
                    "grid": [Ny, Nx], "slices": S, "probe_positions": P, "batch": P,
                    "method": "plain", "schedules": sched},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": d_bytes, "avg_launch_ms": d_ms, "launch_probes": G,
-                     "band_bytes_per_launch": d_band, "band_GBs": d_band / (d_ms * 1e-3) / 1e9,
-                     "band_frac": d_band / (d_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "slice_step_GBs": step_gbs, "slice_step_band_GBs": step_band,
-                     "row_ms": row_ms, "col_ms": col_ms, "copy_GBs": copy_gbs},
+        "roofline": {**rf, "traffic": traffic, "copy_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "finite_outputs": bool(sane),
     }

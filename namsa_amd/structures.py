@@ -10,6 +10,10 @@ here produce the same record layout for the benchmark workloads:
   random_amorphous_crystal  Z drawn from a set, a jittered simple-cubic
                          lattice in the lower half and random sequential
                          addition with a minimum spacing in the upper half
+  random_crystal, multi_material_specs   seeded cubic / tetragonal /
+                         hexagonal crystals for uneven multi-material batches
+  polycrystal            Voronoi grains of Si-, GaAs- and Au-like lattices in
+                         random 3D orientations
 """
 from __future__ import annotations
 
@@ -218,3 +222,81 @@ def multi_material_specs(seed=3, count=48, grids=(128, 192, 256, 384, 512, 768, 
                       "kv": float(rng.uniform(*kv)), "semi": float(rng.uniform(*semi_mrad)) * 1e-3,
                       "grid_steps": (gx, gy), "thickness": T, "lateral": lateral, **info})
     return specs
+
+
+# ---------------------------------------------------------------- polycrystal
+# (name, cubic lattice constant A, basis in fractional coordinates, Z per basis site)
+_FCC = np.array([[0, 0, 0], [0, 0.5, 0.5], [0.5, 0, 0.5], [0.5, 0.5, 0]])
+_GRAIN_LATTICES = (
+    ("Si", 5.431, np.concatenate([_FCC, _FCC + 0.25]), [14] * 4 + [14] * 4),
+    ("GaAs", 5.653, np.concatenate([_FCC, _FCC + 0.25]), [31] * 4 + [33] * 4),
+    ("Au", 4.078, _FCC, [79] * 4),
+)
+
+
+def _random_rotation(rng):
+    """Uniformly distributed rotation matrix (QR of a Gaussian matrix, sign-fixed)."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def polycrystal(seed=5, lateral=128.0, thickness=1000.0, grains=4, slab=50.0, dmin=1.2):
+    """Seeded Voronoi polycrystal filling lateral x lateral x thickness A.
+
+    `grains` seeds are uniform in the box; each grain takes one of the
+    diamond-Si, zincblende-GaAs and fcc-Au lattices (cycled, so 4 grains use
+    all three) in a uniformly random 3D orientation, and keeps the lattice
+    sites whose nearest seed is its own.  Sites are generated slab by slab
+    along z (`slab` A thick) so memory stays bounded.  Across grain
+    boundaries, of two atoms closer than `dmin` the later one is dropped
+    (within a grain the closest pair is 2.35 A).  One atom is pinned at
+    z = thickness so the extent is exact.  Returns (records, info).
+    """
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    box = np.array([lateral, lateral, thickness])
+    seeds = rng.uniform(0, 1, size=(grains, 3)) * box
+    kinds = [_GRAIN_LATTICES[g % len(_GRAIN_LATTICES)] for g in range(grains)]
+    rots = [_random_rotation(rng) for _ in range(grains)]
+    origins = rng.uniform(0, 1, size=(grains, 3))
+    xyz_all, Z_all = [], []
+    for z0 in np.arange(0.0, thickness, slab):
+        z1 = min(thickness, z0 + slab)
+        lo = np.array([0.0, 0.0, z0])
+        hi = np.array([lateral, lateral, z1])
+        corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+        for g in range(grains):
+            name, a, basis, Zb = kinds[g]
+            Rm = rots[g]
+            # lattice frame: p_lab = Rm @ (a (n + b + origin))  =>  n ~ Rm^T p_lab / a
+            f = corners @ Rm / a - origins[g]
+            nlo = np.floor(f.min(0)).astype(int) - 1
+            nhi = np.ceil(f.max(0)).astype(int) + 1
+            grid = np.stack(np.meshgrid(*[np.arange(nlo[i], nhi[i]) for i in range(3)], indexing="ij"),
+                            -1).reshape(-1, 1, 3)
+            frac = (grid + basis[None] + origins[g]).reshape(-1, 3)
+            lab = (frac * a) @ Rm.T
+            inside = np.all((lab >= lo) & (lab < hi), axis=1)
+            lab = lab[inside]
+            zs = np.tile(np.asarray(Zb), len(grid))[inside]
+            d2 = ((lab[:, None, :] - seeds[None]) ** 2).sum(-1)
+            mine = np.argmin(d2, axis=1) == g
+            xyz_all.append(lab[mine])
+            Z_all.append(zs[mine])
+    xyz = np.concatenate(xyz_all)
+    Z = np.concatenate(Z_all)
+    pairs = cKDTree(xyz).query_pairs(r=dmin, output_type="ndarray")
+    if len(pairs):
+        drop = np.zeros(len(xyz), bool)
+        for i, j in pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]:
+            if not drop[i]:
+                drop[j] = True
+        xyz, Z = xyz[~drop], Z[~drop]
+    top = int(np.argmax(xyz[:, 2]))
+    xyz[top, 2] = thickness
+    info = {"grains": [k[0] for k in kinds], "seeds": seeds.tolist(), "atoms": int(len(Z))}
+    return make_records(xyz, Z), info

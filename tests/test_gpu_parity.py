@@ -245,6 +245,25 @@ def test_config2_full_size_parity_two_probes(gpu):
     assert max(errs) < 1e-4
 
 
+def test_polycrystal_2048_parity(gpu):
+    """A thin slab of the seeded 2048^2 polycrystal (grain boundaries, four
+    elements, 300 kV, 30 mrad) against the oracle, one probe near a triple
+    junction region and one in a grain."""
+    from namsa_amd import structures
+
+    _say("polycrystal 2048^2, 6 slices")
+    rec, info = structures.polycrystal(seed=5, thickness=6.0)
+    sim = MSAGPU(300, 0.030, rec, sampling=np.array([2048, 2048]), debye_waller=False)
+    pos = np.array([[0.0, 0.0], [-20.5, 31.25]])
+    _run_sim(sim, 1.0, pos)
+    t_ref, _ = oracle_slices(sim, 1.0)
+    psi_k, _ = oracle_probe(sim)
+    cb_ref = O.cbed(psi_k, pos.astype(np.float32), np.float32(sim.kmax), t_ref, oracle_pm(sim, 1.0, 1 / 3))
+    errs = [rel_l2(sim.probes[i], cb_ref[i]) for i in range(len(pos))]
+    _say(f"  atoms {info['atoms']} S={sim.num_slices} CBED relL2 {['%.2e' % e for e in errs]}")
+    assert max(errs) < 1e-4
+
+
 def test_vacuum_closed_form_2048(gpu):
     """No slices (S=0) and transparent slices both give mask-free / masked |psi_k|^2."""
     import torch
