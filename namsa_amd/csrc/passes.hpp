@@ -44,6 +44,10 @@ struct ColArgs {
   double* det_ws;        // [B][strips_per_image][n_det] (READOUT)
 };
 
+// MI355X has 8 XCDs, each with its own L2; the dispatcher deals workgroups
+// round-robin over them in workgroup-id order.
+constexpr int kXcds = 8;
+
 // Narrowest column tile of any compiled schedule: detector scratch is sized
 // for it so that any schedule choice fits.
 constexpr int kMinColTile = 4;

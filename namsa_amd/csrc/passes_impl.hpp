@@ -172,7 +172,14 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
   // multiple of strips), so the strip's propagator and probe values are read
   // once into registers; the workgroups resident at one time still cover all
   // strips of a few images, which keeps HBM rows contiguous.
-  const int strip = a.strip0 + static_cast<int>(blockIdx.x % a.n_strips);
+  // XCD-aware virtual workgroup id: the dispatcher deals workgroups
+  // round-robin over the XCDs (each with its own L2), so consecutive virtual
+  // ids -- adjacent strips of one image, which share 128-byte lines when the
+  // strips are narrower than a line -- run on one XCD.
+  const int xcd = static_cast<int>(blockIdx.x % kXcds);
+  const int q = static_cast<int>(gridDim.x / kXcds), r = static_cast<int>(gridDim.x % kXcds);
+  const int vid = xcd * q + min(xcd, r) + static_cast<int>(blockIdx.x / kXcds);
+  const int strip = a.strip0 + vid % a.n_strips;
   const int col = strip * C + c;
   const long long step = gridDim.x / a.n_strips;
   // READOUT covers every strip; a strip outside the band has a zero
@@ -203,7 +210,7 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
   };
 
   float2 pv[E];
-  long long b = blockIdx.x / a.n_strips;
+  long long b = vid / a.n_strips;
   if constexpr (S::PF && LOADS) {
     if (b < a.B && live) load(b, pv);
   }
@@ -310,11 +317,22 @@ int launch_col_op(ColArgs& a, hipStream_t st) {
     a.strip0 = 0;
     a.n_strips = a.strips_per_image;
   }
-  // a whole number of strip sets, so that every workgroup keeps one strip
-  const long long sets = std::max<long long>(1, std::min<long long>(a.B, cap / a.n_strips));
-  const int grid = static_cast<int>(sets * a.n_strips);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(S::threads()), 0, st, a);
-  return check_hip(hipGetLastError(), "column pass launch");
+  // Every workgroup keeps one strip for the whole launch, so a launch covers
+  // a whole number of strip sets.  With more strips than resident workgroups
+  // the strips go in chunks of at most `cap`, largest first; each chunk is
+  // its own launch on the same stream.
+  const int strip_end = a.strip0 + a.n_strips;
+  for (int s0 = a.strip0; s0 < strip_end;) {
+    ColArgs c = a;
+    c.strip0 = s0;
+    c.n_strips = std::min(cap, strip_end - s0);
+    const long long sets = std::max<long long>(1, std::min<long long>(c.B, cap / c.n_strips));
+    const int grid = static_cast<int>(sets * c.n_strips);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(S::threads()), 0, st, c);
+    if (int rc = check_hip(hipGetLastError(), "column pass launch")) return rc;
+    s0 += c.n_strips;
+  }
+  return kOk;
 }
 
 using RowLaunch = int (*)(int op, const RowArgs&, hipStream_t);
