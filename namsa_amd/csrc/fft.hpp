@@ -143,6 +143,19 @@ template <int E, int NS, int R, int... Rs>
 struct TwCount<E, NS, R, Rs...> {
   static constexpr int value = (NS > 1 ? (E / R) * (R - 1) : 0) + TwCount<E, NS * R, Rs...>::value;
 };
+// Compact twiddles: only w per butterfly is kept; w^2 .. w^(R-1) are formed
+// by complex multiplication when the stage runs (fewer registers, a few more
+// multiplies; the powers carry a few float32 ulp of rounding).
+template <int E, int NS, int... Rs>
+struct TwCountC;
+template <int E, int NS>
+struct TwCountC<E, NS> {
+  static constexpr int value = 0;
+};
+template <int E, int NS, int R, int... Rs>
+struct TwCountC<E, NS, R, Rs...> {
+  static constexpr int value = (NS > 1 ? E / R : 0) + TwCountC<E, NS * R, Rs...>::value;
+};
 
 // Synchronisation between the LDS write and read phases of one exchange.
 struct WaveSync {  // all threads of one transform live in one wavefront
@@ -185,14 +198,43 @@ struct ColLds {
   static constexpr int padded(int n) { return n * C + (n * C) / 32; }
 };
 
-template <int N, int E, class RL>
+// exp(i theta) as a float pair whose magnitude is as close to 1 as float
+// allows: of the two float neighbours of each of cos and sin, the pair with
+// the smallest |c^2 + s^2 - 1|.  Rounding cos and sin independently leaves a
+// magnitude error of a few 1e-8 that does not average out over a transform's
+// few distinct twiddles, and it compounds over hundreds of slices (and over
+// the powers of the compact-twiddle mode).
+__device__ __forceinline__ float2 unit_twiddle(double c, double s) {
+  const float c0 = float(c), s0 = float(s);
+  const float c1 = double(c0) > c ? nextafterf(c0, -2.0f) : nextafterf(c0, 2.0f);
+  const float s1 = double(s0) > s ? nextafterf(s0, -2.0f) : nextafterf(s0, 2.0f);
+  float2 best = make_float2(c0, s0);
+  double err = fabs(double(c0) * c0 + double(s0) * s0 - 1.0);
+  const float cs[2] = {c0, c1}, ss[2] = {s0, s1};
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double e = fabs(double(cs[i]) * cs[i] + double(ss[j]) * ss[j] - 1.0);
+      if (e < err) {
+        err = e;
+        best = make_float2(cs[i], ss[j]);
+      }
+    }
+  return best;
+}
+
+template <int N, int E, class RL, bool CT = false>
 struct Fft;
 
-template <int N, int E, int... Rs>
-struct Fft<N, E, Radices<Rs...>> {
+template <int N, int E, bool CT, int... Rs>
+struct Fft<N, E, Radices<Rs...>, CT> {
   static_assert(Prod<Rs...>::value == N, "radix schedule must multiply to N");
   static constexpr int T = N / E;
-  static constexpr int NTW = TwCount<E, 1, Rs...>::value;
+  static constexpr int NTW = CT ? TwCountC<E, 1, Rs...>::value : TwCount<E, 1, Rs...>::value;
+  // registers per butterfly of a twiddled stage
+  template <int R>
+  static constexpr int per_bfly() { return CT ? 1 : R - 1; }
   static constexpr int NTW_ALLOC = NTW > 0 ? NTW : 1;
   static constexpr int NSTAGES = sizeof...(Rs);
 
@@ -203,18 +245,19 @@ struct Fft<N, E, Radices<Rs...>> {
   template <int NS, int OFF, int R, int... Rest>
   static __device__ __forceinline__ void tw_stages(int t, float2 (&tw)[NTW_ALLOC]) {
     constexpr int Q = E / R;
+    constexpr int P = per_bfly<R>();
     if constexpr (NS > 1) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int k = (t + T * q) % NS;
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
+        for (int r = 1; r <= P; ++r) {
           double s, c;
           sincospi(-2.0 * double(r * k) / double(NS * R), &s, &c);
-          tw[OFF + q * (R - 1) + (r - 1)] = make_float2(float(c), float(s));
+          tw[OFF + q * P + (r - 1)] = unit_twiddle(c, s);
         }
       }
-      tw_stages<NS * R, OFF + Q * (R - 1), Rest...>(t, tw);
+      tw_stages<NS * R, OFF + Q * P, Rest...>(t, tw);
     } else {
       tw_stages<NS * R, OFF, Rest...>(t, tw);
     }
@@ -236,10 +279,24 @@ struct Fft<N, E, Radices<Rs...>> {
 #pragma unroll
       for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
       if constexpr (NS > 1) {
+        if constexpr (CT) {
+          float2 w1 = tw[OFF + q];
+          // Opaque to the optimiser (no instruction is emitted): keeps the
+          // powers of w1 from being hoisted out of the caller's persistent
+          // loop, which would bring back one register pair per power.
+          asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+          float2 w = w1;
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          const float2 w = tw[OFF + q * (R - 1) + (r - 1)];
-          a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+          for (int r = 1; r < R; ++r) {
+            a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+            if (r + 1 < R) w = cmul(w, w1);
+          }
+        } else {
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            const float2 w = tw[OFF + q * (R - 1) + (r - 1)];
+            a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+          }
         }
       }
       Dft<R, INV>::run(a);
@@ -273,7 +330,7 @@ struct Fft<N, E, Radices<Rs...>> {
   static __device__ __forceinline__ void stages(float2 (&v)[E], Lds& lds, int t,
                                                 const float2 (&tw)[NTW_ALLOC]) {
     stage<INV, R, NS, OFF, sizeof...(Rest) == 0, Lds, Sync>(v, lds, t, tw);
-    constexpr int NEXT_OFF = OFF + (NS > 1 ? (E / R) * (R - 1) : 0);
+    constexpr int NEXT_OFF = OFF + (NS > 1 ? (E / R) * per_bfly<R>() : 0);
     stages<INV, NS * R, NEXT_OFF, Lds, Sync, Rest...>(v, lds, t, tw);
   }
 

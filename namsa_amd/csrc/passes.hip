@@ -34,8 +34,8 @@ static const SchedTable* table_for(int n) {
 // the cache-sized probe group (3/4 of the 256 MB LLC of waves per launch),
 // timed as back-to-back slice steps.
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = {1,  0,  2,  0,  0,  0,  0,   0};
-static const int kDefaultCol[kNumSizes] = {0,  0,  2,  1, 10,  1,  1,   2};
+static const int kDefaultRow[kNumSizes] = { 1,  0,  2,  2,  5,  2,  3,  2};
+static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  2, 13,  2,  3,  3};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];

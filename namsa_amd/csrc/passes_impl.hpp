@@ -44,19 +44,19 @@ __device__ __forceinline__ void st(float2* p, float2 v) {
   }
 }
 
-template <int N_, int E_, class RL_, bool PF_, bool NT_>
+template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false>
 struct RowS {
   static constexpr int N = N_, E = E_;
   using RL = RL_;
-  static constexpr bool PF = PF_, NT = NT_;
+  static constexpr bool PF = PF_, NT = NT_, CT = CT_;
 };
 
-template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false>
+template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false, bool CT_ = false>
 struct ColS {
   static_assert(C_ >= kMinColTile && C_ <= kMaxColTile, "column tile width out of range");
   static constexpr int N = N_, E = E_, C = C_;
   using RL = RL_;
-  static constexpr bool PF = PF_, NT = NT_, DB = DB_;
+  static constexpr bool PF = PF_, NT = NT_, DB = DB_, CT = CT_;
   static constexpr int threads() { return C * (N / E); }
 };
 
@@ -72,7 +72,7 @@ __device__ __forceinline__ float2 unit_phase(double turns) {
 template <class S, int OP>
 __global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
   constexpr int N = S::N, E = S::E;
-  using F = Fft<N, E, typename S::RL>;
+  using F = Fft<N, E, typename S::RL, S::CT>;
   constexpr int T = F::T;
   constexpr int G = kRowThreads / T;
   constexpr bool STEP = OP == ROW_STEP;
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
 template <class S, int OP>
 __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
   constexpr int N = S::N, E = S::E, C = S::C;
-  using F = Fft<N, E, typename S::RL>;
+  using F = Fft<N, E, typename S::RL, S::CT>;
   constexpr int T = F::T;
   constexpr int NW = S::threads() / 64;
   constexpr bool LOADS = OP != COL_PROBE;

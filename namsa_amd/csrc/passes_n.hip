@@ -1,7 +1,8 @@
 // Compiled schedules of the row and column passes for one transform length.
 // Built once per length with -DNAMSA_PASS_N=<n>; entry 0 is the base schedule.
 // Names read: e<E> = elements per thread, r<radices>, c<C> = column tile,
-// pf = next row/tile prefetched, nt = non-temporal wave traffic.
+// pf = next row/tile prefetched, nt = non-temporal wave traffic, ct = compact
+// twiddles (one register pair per butterfly, higher powers formed in the stage).
 #include "passes_impl.hpp"
 
 #ifndef NAMSA_PASS_N
@@ -44,9 +45,16 @@ using R0 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, false, false>;
 using R1 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, true, false>;
 using C0 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 16, false, false>;
 using C1 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false>;
-static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>}, {"e12.r32444.pf", launch_row<R1, R0>}};
+using R2 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, false, false, true>;
+using C2 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, false, true>;
+using C3 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 16, true, false, false, true>;
+static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>},
+                                      {"e12.r32444.pf", launch_row<R1, R0>},
+                                      {"e12.r32444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r32444.c16", launch_col<C0, C0>, 16},
-                                      {"e12.r32444.c8.pf", launch_col<C1, C0>, 8}};
+                                      {"e12.r32444.c8.pf", launch_col<C1, C0>, 8},
+                                      {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8},
+                                      {"e12.r32444.c16.pf.ct", launch_col<C3, C0>, 16}};
 #elif NAMSA_PASS_N == 512
 using R0 = RowS<512, 8, Radices<8, 8, 8>, false, false>;
 using R1 = RowS<512, 8, Radices<8, 8, 8>, true, false>;
@@ -66,11 +74,14 @@ using C9 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, true>;
 using C10 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false>;
 using C11 = ColS<512, 16, Radices<8, 8, 8>, 16, true, false, true>;
 using C12 = ColS<512, 16, Radices<16, 16, 2>, 16, true, false>;
+using C13 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, false, true>;
+using R5 = RowS<512, 8, Radices<8, 8, 8>, false, false, true>;
 static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
                                       {"e8.r888.pf", launch_row<R1, R0>},
                                       {"e8.r888.pf.nt", launch_row<R2, R0>},
                                       {"e16.r16.16.2", launch_row<R3, R0>},
-                                      {"e16.r16.16.2.pf", launch_row<R4, R0>}};
+                                      {"e16.r16.16.2.pf", launch_row<R4, R0>},
+                                      {"e8.r888.ct", launch_row<R5, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e8.r888.c8", launch_col<C1, C0>, 8},
                                       {"e8.r888.c8.pf", launch_col<C2, C0>, 8},
@@ -83,39 +94,59 @@ static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e8.r888.c16.db.pf", launch_col<C9, C0>, 16},
                                       {"e8.r888.c16.pf", launch_col<C10, C0>, 16},
                                       {"e16.r888.c16.db.pf", launch_col<C11, C0>, 16},
-                                      {"e16.r16.16.2.c16.pf", launch_col<C12, C0>, 16}};
+                                      {"e16.r16.16.2.c16.pf", launch_col<C12, C0>, 16},
+                                      {"e8.r888.c16.pf.ct", launch_col<C13, C0>, 16}};
 #elif NAMSA_PASS_N == 768
 using R0 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false>;
 using R1 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, true, false>;
 using C0 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false>;
 using C1 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false>;
-static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>}, {"e12.r34444.pf", launch_row<R1, R0>}};
+using R2 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false, true>;
+using C2 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, false, true>;
+using C4 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 16, true, false, false, true>;
+static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>},
+                                      {"e12.r34444.pf", launch_row<R1, R0>},
+                                      {"e12.r34444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
-                                      {"e12.r34444.c8.pf", launch_col<C1, C0>, 8}};
+                                      {"e12.r34444.c8.pf", launch_col<C1, C0>, 8},
+                                      {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8},
+                                      {"e12.r34444.c16.pf.ct", launch_col<C4, C0>, 16}};
 #elif NAMSA_PASS_N == 1024
 using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
 using R2 = RowS<1024, 8, Radices<8, 8, 4, 4>, true, false>;
+using R3 = RowS<1024, 16, Radices<16, 8, 8>, false, false, true>;
 using C0 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false>;
 using C1 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false>;
 using C2 = ColS<1024, 8, Radices<8, 8, 4, 4>, 8, false, false>;
 static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
                                       {"e16.r16.8.8.pf", launch_row<R1, R0>},
-                                      {"e8.r8844.pf", launch_row<R2, R0>}};
+                                      {"e8.r8844.pf", launch_row<R2, R0>},
+                                      {"e16.r16.8.8.ct", launch_row<R3, R0>}};
+using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
+using C4 = ColS<1024, 16, Radices<16, 8, 8>, 16, true, false, false, true>;
 static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
                                       {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
-                                      {"e8.r8844.c8", launch_col<C2, C0>, 8}};
+                                      {"e8.r8844.c8", launch_col<C2, C0>, 8},
+                                      {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8},
+                                      {"e16.r16.8.8.c16.pf.ct", launch_col<C4, C0>, 16}};
 #elif NAMSA_PASS_N == 2048
 using R0 = RowS<2048, 16, Radices<16, 16, 8>, false, false>;
 using R1 = RowS<2048, 16, Radices<16, 16, 8>, true, false>;
 using C0 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false>;
 using C1 = ColS<2048, 16, Radices<16, 16, 8>, 8, false, false>;
 using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
+using R2 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true>;
+using C3 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false, false, true>;
+using C4 = ColS<2048, 16, Radices<16, 16, 8>, 8, true, false, false, true>;
 static const RowSchedEntry kRows[] = {{"e16.r16.16.8", launch_row<R0, R0>},
-                                      {"e16.r16.16.8.pf", launch_row<R1, R0>}};
+                                      {"e16.r16.16.8.pf", launch_row<R1, R0>},
+                                      {"e16.r16.16.8.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r16.16.8.c4", launch_col<C0, C0>, 4},
                                       {"e16.r16.16.8.c8", launch_col<C1, C0>, 8},
-                                      {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4}};
+                                      {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4},
+                                      {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4},
+                                      {"e16.r16.16.8.c8.pf.ct", launch_col<C4, C0>, 8}};
 #else
 #error "no schedules for this transform length"
 #endif
