@@ -4,7 +4,8 @@ Too long for the test suite (the FP64 oracle needs ~2000 FFTs of 2048^2), so
 it runs in two steps:
 
   on the GPU:   python tests/full_size_config5.py gpu  gpurun_out/c5_full.npz
-  on any CPU:   python tests/full_size_config5.py oracle gpurun_out/c5_full.npz
+  on any CPU:   python tests/full_size_config5.py oracle gpurun_out/c5_full.npz [oracle.npz]
+  later runs:   python tests/full_size_config5.py compare gpurun_out/c5_full.npz oracle.npz
 
 The first writes the GPU CBEDs of two probe positions (MSAGPU, dz = 1 A,
 bandwidth 1/3); the second rebuilds the same structure, streams the oracle
@@ -49,7 +50,17 @@ def run_gpu(out):
     np.savez_compressed(out, cbed=cb, pos=POS, slices=int(sim.num_slices))
 
 
-def run_oracle(path):
+def compare(cb_gpu, cb):
+    for b in range(len(POS)):
+        e = np.linalg.norm(cb_gpu[b] - cb[b]) / np.linalg.norm(cb[b])
+        print(f"position {POS[b]}: CBED rel-L2 {e:.3e}  (sum GPU {cb_gpu[b].sum():.6f}, oracle {cb[b].sum():.6f})")
+
+
+def run_compare(path, oracle_path):
+    compare(np.load(path)["cbed"], np.load(oracle_path)["cbed"])
+
+
+def run_oracle(path, save=None):
     from oracle import msa_oracle as O
 
     data = np.load(path)
@@ -74,10 +85,10 @@ def run_oracle(path):
         if i % 100 == 0:
             print(f"oracle slice {i}/{len(sites)} ({time.time() - t0:.0f} s)", flush=True)
     cb = O.readout(psi, norm)
-    for b in range(len(POS)):
-        e = np.linalg.norm(cb_gpu[b] - cb[b]) / np.linalg.norm(cb[b])
-        print(f"position {POS[b]}: CBED rel-L2 {e:.3e}  (sum GPU {cb_gpu[b].sum():.6f}, oracle {cb[b].sum():.6f})")
+    if save:
+        np.savez_compressed(save, cbed=cb, pos=POS)
+    compare(cb_gpu, cb)
 
 
 if __name__ == "__main__":
-    {"gpu": run_gpu, "oracle": run_oracle}[sys.argv[1]](sys.argv[2])
+    {"gpu": run_gpu, "oracle": run_oracle, "compare": run_compare}[sys.argv[1]](*sys.argv[2:])
