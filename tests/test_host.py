@@ -56,6 +56,28 @@ def test_si111_structure():
     assert d[:, 1].min() == pytest.approx(5.431 * np.sqrt(3) / 4, rel=1e-4)  # Si-Si bond
 
 
+def test_polycrystal_structure():
+    """Seeded Voronoi polycrystal: deterministic, fills the box, grain lattices
+    keep their nearest-neighbour distances, boundaries keep >= dmin."""
+    from scipy.spatial import cKDTree
+
+    rec, info = structures.polycrystal(seed=5, lateral=40.0, thickness=30.0, slab=10.0)
+    rec2, _ = structures.polycrystal(seed=5, lateral=40.0, thickness=30.0, slab=10.0)
+    np.testing.assert_array_equal(rec, rec2)
+    xyz = np.stack([rec["x"], rec["y"], rec["z"]], 1).astype(np.float64)
+    assert info["grains"] == ["Si", "GaAs", "Au", "Si"]
+    assert set(np.unique(rec["atomic_number"])) <= {14, 31, 33, 79}
+    assert xyz.min() >= 0 and xyz[:, :2].max() < 40.0 and xyz[:, 2].max() == pytest.approx(30.0)
+    d, _ = cKDTree(xyz).query(xyz, k=2)
+    assert d[:, 1].min() >= 1.2 - 1e-4
+    # density between Au-fcc (0.059) and GaAs (0.044) per cubic Angstrom
+    assert 0.04 < len(rec) / (40 * 40 * 30) < 0.065
+    # inside Au grains the fcc nearest-neighbour distance a/sqrt(2) dominates
+    au = xyz[rec["atomic_number"] == 79]
+    da, _ = cKDTree(au).query(au, k=2)
+    assert np.median(da[:, 1]) == pytest.approx(4.078 / np.sqrt(2), rel=1e-4)
+
+
 def test_site_table_matches_oracle_binning(config2):
     sim = _sim(config2, N=512)
     sim.unique_Z = np.unique(sim.supercell_Z)
