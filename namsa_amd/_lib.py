@@ -74,6 +74,11 @@ def load() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise NamsaError(f"{LIB_PATH} is not built; run `python -m namsa_amd.build`")
+    # torch ships its own HIP runtime; it must be the one in the process before
+    # the library binds libamdhip64 (which then resolves to that same copy, so
+    # torch's streams and allocations are valid here).  Loading the library
+    # first would bind the system runtime and leave torch without devices.
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
