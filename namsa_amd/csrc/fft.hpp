@@ -20,6 +20,31 @@
 namespace namsa {
 
 // ------------------------------------------------------------------ radix-R DFTs
+// Butterfly constants are carried as float pairs hi + lo (the double value to
+// ~1e-15) and applied with one fused multiply-add: x * hi + (x * lo).  The
+// nearest floats alone have magnitude errors of 3e-8..6e-8 (2 fl(sqrt(1/2))^2
+// = 1 - 3.4e-8, fl(cos pi/8)^2 + fl(sin pi/8)^2 = 1 - 5.7e-8) that shrink every
+// rotated element the same way and compound coherently over hundreds of
+// slices (-2.4e-7 of the intensity per slice at 2048^2); with the pairs only
+// the unbiased rounding of each product remains.
+struct KPair {
+  float hi, lo;
+};
+constexpr KPair kR2{0.70710676908493042f, 1.2101617485882343e-08f};   // sqrt(1/2)
+constexpr KPair kC8{0.92387950420379639f, 2.830748968563057e-08f};    // cos(pi/8)
+constexpr KPair kS8{0.38268342614173889f, 6.2233507236442165e-09f};   // sin(pi/8)
+constexpr KPair kR3{0.86602538824081421f, 1.5543625053737742e-08f};   // sqrt(3)/2
+
+__device__ __forceinline__ float kmul(float x, KPair k) { return fmaf(x, k.hi, x * k.lo); }
+
+// (x + i y) * (c + i s) with c, s given as pairs
+__device__ __forceinline__ float2 kcmul(float2 v, KPair c, KPair s, bool neg_s) {
+  const float sh = neg_s ? -s.hi : s.hi, sl = neg_s ? -s.lo : s.lo;
+  const float re = fmaf(v.x, c.hi, fmaf(-v.y, sh, fmaf(v.x, c.lo, -v.y * sl)));
+  const float im = fmaf(v.x, sh, fmaf(v.y, c.hi, fmaf(v.x, sl, v.y * c.lo)));
+  return make_float2(re, im);
+}
+
 template <int R, bool INV>
 struct Dft;
 
@@ -35,9 +60,9 @@ struct Dft<2, INV> {
 template <bool INV>
 struct Dft<3, INV> {
   static __device__ __forceinline__ void run(float2* a) {
-    const float s = 0.86602540378443865f;  // sqrt(3)/2
     float2 t1 = cadd(a[1], a[2]);
-    float2 t2 = cscale(csub(a[1], a[2]), s);
+    const float2 d = csub(a[1], a[2]);
+    float2 t2 = make_float2(kmul(d.x, kR3), kmul(d.y, kR3));
     float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
     a[0] = cadd(a[0], t1);
     float2 r = mul_mi<INV>(t2);
@@ -63,18 +88,17 @@ struct Dft<4, INV> {
 template <bool INV>
 struct Dft<8, INV> {
   static __device__ __forceinline__ void run(float2* a) {
-    const float c = 0.70710678118654752f;
     float2 e[4] = {a[0], a[2], a[4], a[6]};
     float2 o[4] = {a[1], a[3], a[5], a[7]};
     Dft<4, INV>::run(e);
     Dft<4, INV>::run(o);
     // o[k] *= exp(-+ 2 pi i k / 8)
     if (INV) {
-      o[1] = make_float2(c * (o[1].x - o[1].y), c * (o[1].x + o[1].y));
-      o[3] = make_float2(-c * (o[3].x + o[3].y), c * (o[3].x - o[3].y));
+      o[1] = make_float2(kmul(o[1].x - o[1].y, kR2), kmul(o[1].x + o[1].y, kR2));
+      o[3] = make_float2(-kmul(o[3].x + o[3].y, kR2), kmul(o[3].x - o[3].y, kR2));
     } else {
-      o[1] = make_float2(c * (o[1].x + o[1].y), c * (o[1].y - o[1].x));
-      o[3] = make_float2(c * (o[3].y - o[3].x), -c * (o[3].x + o[3].y));
+      o[1] = make_float2(kmul(o[1].x + o[1].y, kR2), kmul(o[1].y - o[1].x, kR2));
+      o[3] = make_float2(kmul(o[3].y - o[3].x, kR2), -kmul(o[3].x + o[3].y, kR2));
     }
     o[2] = mul_mi<INV>(o[2]);
 #pragma unroll
@@ -88,11 +112,10 @@ struct Dft<8, INV> {
 template <bool INV>
 struct Dft<16, INV> {
   static __device__ __forceinline__ void run(float2* a) {
-    // cos / sin of k*pi/8, k = 0..7
-    const float C[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
-                        0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
-    const float S[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
-                        1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+    // cos / sin of k*pi/8, k = 0..7, as constant pairs (k = 0, 4 are exact)
+    constexpr KPair Z{0.0f, 0.0f};
+    const KPair C[8] = {Z, kC8, kR2, kS8, Z, {-kS8.hi, -kS8.lo}, {-kR2.hi, -kR2.lo}, {-kC8.hi, -kC8.lo}};
+    const KPair S[8] = {Z, kS8, kR2, kC8, Z, kC8, kR2, kS8};
     float2 e[8], o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -103,11 +126,10 @@ struct Dft<16, INV> {
     Dft<8, INV>::run(o);
 #pragma unroll
     for (int k = 1; k < 8; ++k) {
-      float2 w = make_float2(C[k], INV ? S[k] : -S[k]);
       if (k == 4)
         o[k] = mul_mi<INV>(o[k]);
       else
-        o[k] = cmul(o[k], w);
+        o[k] = kcmul(o[k], C[k], S[k], !INV);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -152,9 +174,13 @@ template <int E, int NS>
 struct TwCountC<E, NS> {
   static constexpr int value = 0;
 };
+// registers per butterfly in compact mode: w for R <= 8; w, w^2, w^4, w^8 for
+// R = 16 (a 14-multiply power chain drifts the magnitude by ~1e-7 per slice)
+constexpr int compact_per_bfly(int R) { return R >= 16 ? 4 : 1; }
 template <int E, int NS, int R, int... Rs>
 struct TwCountC<E, NS, R, Rs...> {
-  static constexpr int value = (NS > 1 ? E / R : 0) + TwCountC<E, NS * R, Rs...>::value;
+  static constexpr int value =
+      (NS > 1 ? (E / R) * compact_per_bfly(R) : 0) + TwCountC<E, NS * R, Rs...>::value;
 };
 
 // Synchronisation between the LDS write and read phases of one exchange.
@@ -234,7 +260,10 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   static constexpr int NTW = CT ? TwCountC<E, 1, Rs...>::value : TwCount<E, 1, Rs...>::value;
   // registers per butterfly of a twiddled stage
   template <int R>
-  static constexpr int per_bfly() { return CT ? 1 : R - 1; }
+  static constexpr int per_bfly() { return CT ? compact_per_bfly(R) : R - 1; }
+  // exponent of the r-th stored twiddle of a butterfly
+  template <int R>
+  static constexpr int stored_power(int r) { return CT && R >= 16 ? 1 << (r - 1) : r; }
   static constexpr int NTW_ALLOC = NTW > 0 ? NTW : 1;
   static constexpr int NSTAGES = sizeof...(Rs);
 
@@ -253,7 +282,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
 #pragma unroll
         for (int r = 1; r <= P; ++r) {
           double s, c;
-          sincospi(-2.0 * double(r * k) / double(NS * R), &s, &c);
+          sincospi(-2.0 * double(stored_power<R>(r) * k) / double(NS * R), &s, &c);
           tw[OFF + q * P + (r - 1)] = unit_twiddle(c, s);
         }
       }
@@ -279,7 +308,25 @@ struct Fft<N, E, Radices<Rs...>, CT> {
 #pragma unroll
       for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
       if constexpr (NS > 1) {
-        if constexpr (CT) {
+        if constexpr (CT && R >= 16) {
+          // stored w, w^2, w^4, w^8; w^r = w^h w^(r-h), h the top bit of r
+          float2 pw[R];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            float2 x = tw[OFF + q * 4 + j];
+            // opaque to the optimiser (no instruction is emitted): keeps the
+            // products from being hoisted out of the persistent loop
+            asm volatile("" : "+v"(x.x), "+v"(x.y));
+            pw[1 << j] = x;
+          }
+#pragma unroll
+          for (int r = 3; r < R; ++r) {
+            const int h = r >= 8 ? 8 : (r >= 4 ? 4 : 2);
+            if (r != h) pw[r] = cmul(pw[h], pw[r - h]);
+          }
+#pragma unroll
+          for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], pw[r]) : cmul(a[r], pw[r]);
+        } else if constexpr (CT) {
           float2 w1 = tw[OFF + q];
           // Opaque to the optimiser (no instruction is emitted): keeps the
           // powers of w1 from being hoisted out of the caller's persistent
