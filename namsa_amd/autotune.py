@@ -78,7 +78,7 @@ def _time(fn, reps: int) -> float:
 
 
 def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbose: bool = False,
-         band=None) -> dict:
+         band=None, compact: bool = False) -> dict:
     """Select the fastest schedule of each pass at n x n with `batch` waves.
 
     Passes are timed the way the slice loop runs them: `reps` slice steps
@@ -89,6 +89,9 @@ def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbo
 
     batch: waves per launch (default: the cache-sized probe group production uses).
     band: the propagator's open column range (default: the 1/3-bandwidth band).
+    compact: also consider the compact-twiddle (.ct) schedules, a few percent
+    faster but with round-off that grows ~3.6x faster with the slice count
+    (1.6e-4 of the CBED after 1000 slices against 4.5e-5).
     """
     import torch
 
@@ -113,6 +116,8 @@ def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbo
 
     for pass_, key in ((PASS_COL, "col"), (PASS_ROW, "row")):
         for v, name in enumerate(schedules(n, pass_)):
+            if name.endswith(".ct") and not compact:
+                continue
             select(n, pass_, v)
             result[key][name] = _time(step, reps)
             if verbose:

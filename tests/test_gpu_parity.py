@@ -246,6 +246,45 @@ def test_config2_full_size_parity_two_probes(gpu):
     assert max(errs) < 1e-4
 
 
+@pytest.mark.parametrize("N,S,bound", [(256, 300, 2e-5), (384, 300, 4.5e-5), (512, 1000, 6e-5),
+                                       (768, 300, 4.5e-5), (1024, 300, 2e-5), (2048, 1000, 6e-5)])
+def test_vacuum_round_off_growth(gpu, N, S, bound):
+    """Size-independent property over thickness: in vacuum (t = 1) with the
+    band limit off every slice is exactly unitary and diagonal in k, so the
+    CBED after S slices equals the incident one; the difference is the
+    accumulated round-off of the default schedules.  It grows linearly: about
+    4.5e-8 per slice for the 2-3 stage power-of-two transforms (1000 slices
+    stay well inside the 1e-4 budget) and about 1.1e-7 per slice for the
+    4-5 stage radix-3 lengths (384, 768)."""
+    import torch
+
+    from namsa_amd._lib import call, ptr, stream_handle
+
+    _say(f"vacuum round-off N={N} S={S}")
+    kmax, lam = 12.0, 0.0197
+    st = stream_handle()
+    psi_k = torch.empty((N, N), dtype=torch.complex64, device="cuda")
+    ap = torch.empty((N, N), device="cuda")
+    call("namsa_probe_k", N, N, np.float32(kmax), np.float32(0.030 / lam), np.float32(lam), np.float32(0),
+         np.float32(0), np.float32(0), 1, np.float32(80.0), ptr(psi_k), ptr(ap), st)
+    pm = torch.empty_like(psi_k)
+    mask = torch.empty((N, N), device="cuda")
+    call("namsa_propagator", N, N, np.float32(kmax), np.float32(10 * kmax), np.float32(lam), np.float32(1.0),
+         ptr(pm), ptr(mask), st)
+    t = torch.ones((S, N, N), dtype=torch.complex64, device="cuda")
+    B = 2
+    pos = torch.tensor([[0.0, 0.0], [3.3, -7.1]], dtype=torch.float32, device="cuda")
+    ws = torch.empty((B, N, N), dtype=torch.complex64, device="cuda")
+    cb = torch.empty((2, B, N, N), device="cuda")
+    for i, s in enumerate((0, S)):
+        call("namsa_multislice", ptr(psi_k), ptr(pos), B, N, N, np.float32(kmax), ptr(t), s, ptr(pm), 0, N, B,
+             L.OUT_CBED, np.float32(1.0), ptr(ws), ptr(cb[i]), None, 0, None, None, st)
+    c = cb.double().cpu().numpy()
+    err = max(rel_l2(c[1, b], c[0, b]) for b in range(B))
+    _say(f"  CBED change after {S} vacuum slices: {err:.2e}")
+    assert err < bound
+
+
 def test_polycrystal_2048_parity(gpu):
     """A thin slab of the seeded 2048^2 polycrystal (grain boundaries, four
     elements, 300 kV, 30 mrad) against the oracle, one probe near a triple
