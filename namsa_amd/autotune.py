@@ -78,7 +78,7 @@ def _time(fn, reps: int) -> float:
 
 
 def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbose: bool = False,
-         band=None, compact: bool = False) -> dict:
+         band=None, compact: bool = True) -> dict:
     """Select the fastest schedule of each pass at n x n with `batch` waves.
 
     Passes are timed the way the slice loop runs them: `reps` slice steps
@@ -89,9 +89,9 @@ def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbo
 
     batch: waves per launch (default: the cache-sized probe group production uses).
     band: the propagator's open column range (default: the 1/3-bandwidth band).
-    compact: also consider the compact-twiddle (.ct) schedules, a few percent
-    faster but with round-off that grows ~3.6x faster with the slice count
-    (1.6e-4 of the CBED after 1000 slices against 4.5e-5).
+    compact: also consider the compact-twiddle (.ct) schedules (fewer
+    registers, 9-27% faster; their round-off grows 1.2-1.4x as fast with the
+    slice count as that of the schedules holding every twiddle).
     """
     import torch
 

@@ -174,9 +174,15 @@ template <int E, int NS>
 struct TwCountC<E, NS> {
   static constexpr int value = 0;
 };
-// registers per butterfly in compact mode: w for R <= 8; w, w^2, w^4, w^8 for
-// R = 16 (a 14-multiply power chain drifts the magnitude by ~1e-7 per slice)
-constexpr int compact_per_bfly(int R) { return R >= 16 ? 4 : 1; }
+// registers per butterfly in compact mode: for R >= 8 the powers of two w,
+// w^2, .. w^(R/2) (log2 R pairs), the others products of at most three of
+// them; for R <= 4 only w (w^2, w^3 by a two-multiply chain).  A
+// forward and an inverse transform multiply to |w^r|^2 per twiddle, so the
+// magnitude error of the formed powers is what accumulates over the slices:
+// with at most two products per power it stays ~1.5x that of directly
+// rounded twiddles (a 1-pair power chain is ~3x and grows to 1.6e-4 of the
+// CBED after 1000 slices).
+constexpr int compact_per_bfly(int R) { return R >= 16 ? 4 : (R >= 8 ? 3 : 1); }
 template <int E, int NS, int R, int... Rs>
 struct TwCountC<E, NS, R, Rs...> {
   static constexpr int value =
@@ -263,7 +269,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   static constexpr int per_bfly() { return CT ? compact_per_bfly(R) : R - 1; }
   // exponent of the r-th stored twiddle of a butterfly
   template <int R>
-  static constexpr int stored_power(int r) { return CT && R >= 16 ? 1 << (r - 1) : r; }
+  static constexpr int stored_power(int r) { return CT && R >= 8 ? 1 << (r - 1) : r; }
   static constexpr int NTW_ALLOC = NTW > 0 ? NTW : 1;
   static constexpr int NSTAGES = sizeof...(Rs);
 
@@ -308,14 +314,17 @@ struct Fft<N, E, Radices<Rs...>, CT> {
 #pragma unroll
       for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
       if constexpr (NS > 1) {
-        if constexpr (CT && R >= 16) {
-          // stored w, w^2, w^4, w^8; w^r = w^h w^(r-h), h the top bit of r
+        if constexpr (CT && R >= 8) {
+          // stored w^(2^j); w^r = w^h w^(r-h), h the top bit of r
+          static_assert((R & (R - 1)) == 0, "compact twiddles need a power-of-two radix");
+          constexpr int P = compact_per_bfly(R);
           float2 pw[R];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            float2 x = tw[OFF + q * 4 + j];
+          for (int j = 0; j < P; ++j) {
+            float2 x = tw[OFF + q * P + j];
             // opaque to the optimiser (no instruction is emitted): keeps the
-            // products from being hoisted out of the persistent loop
+            // products from being hoisted out of the persistent loop, which
+            // would bring back one register pair per power
             asm volatile("" : "+v"(x.x), "+v"(x.y));
             pw[1 << j] = x;
           }
@@ -328,10 +337,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
           for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], pw[r]) : cmul(a[r], pw[r]);
         } else if constexpr (CT) {
           float2 w1 = tw[OFF + q];
-          // Opaque to the optimiser (no instruction is emitted): keeps the
-          // powers of w1 from being hoisted out of the caller's persistent
-          // loop, which would bring back one register pair per power.
-          asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+          asm volatile("" : "+v"(w1.x), "+v"(w1.y));  // see above: no hoisting
           float2 w = w1;
 #pragma unroll
           for (int r = 1; r < R; ++r) {

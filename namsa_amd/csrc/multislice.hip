@@ -309,6 +309,7 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
     // band; strips are at most kStoreAlign wide and aligned to their width,
     // so the row pass stores just the kStoreAlign-aligned band.
     const int last_prop = (mode == NAMSA_OUT_EXIT) ? S : S - 1;
+    const bool precise = S > kPreciseSlices;
     const int store_lo = band_lo / kStoreAlign * kStoreAlign;
     const int store_hi = std::min(Nx, (band_hi + kStoreAlign - 1) / kStoreAlign * kStoreAlign);
     for (int i = 0; i < S; ++i) {
@@ -316,9 +317,9 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
       RowArgs r = row_args(psi, Bg, Ny, Nx, tt + i * plane, 1.0f, lo, hi);
       r.store_lo = store_lo;
       r.store_hi = store_hi;
-      if (int rc = row_pass(Nx, ROW_STEP, r, st)) return rc;
+      if (int rc = row_pass(Nx, ROW_STEP, r, st, precise)) return rc;
       if (i < last_prop)
-        if (int rc = col_pass(Ny, COL_PROP, a, st)) return rc;
+        if (int rc = col_pass(Ny, COL_PROP, a, st, precise)) return rc;
     }
     if (mode == NAMSA_OUT_EXIT) {
       const int lo = S == 0 ? 0 : band_lo, hi = S == 0 ? Nx : band_hi;

@@ -1,8 +1,10 @@
 // Schedule registry: which compiled row / column schedule each transform
 // length uses, and the C ABI to list and select them (the autotuner's knob).
 #include <atomic>
+#include <cstring>
 
 #include "namsa_hip.h"
+
 #include "passes_impl.hpp"
 
 namespace namsa {
@@ -32,13 +34,14 @@ static const SchedTable* table_for(int n) {
 
 // Default schedule per length and pass: the fastest schedules measured on
 // MI355X at the cache-sized probe group (3/4 of the 256 MB LLC of waves per
-// launch, timed as back-to-back slice steps) among those whose round-off
-// stays below 5e-5 of the CBED after 1000 vacuum slices.  The compact-twiddle
-// (.ct) schedules are 3-7% faster but their power products add a coherent
-// phase error per slice (1.6e-4 after 1000 slices), so they are opt-in.
+// launch, timed as back-to-back slice steps).  For 512 and up these are the
+// compact-twiddle (.ct) schedules, 9-11% faster than the ones holding every
+// twiddle, with 1.2-1.35x their round-off (about 6e-5 of the CBED after 1000
+// vacuum slices).  The radix-3 lengths keep every twiddle: with 4-5 stages
+// their round-off is already 3.5e-5 after 300 slices, 6.2e-5 in compact form.
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 1,  0,  2,  0,  0,  0,  0,  0};
-static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  1, 10,  1,  1,  2};
+static const int kDefaultRow[kNumSizes] = { 1,  0,  2,  0,  5,  0,  3,  2};
+static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  1, 13,  1,  3,  3};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];
@@ -56,21 +59,35 @@ static void init_selection() {
 
 bool supported(int n) { return size_index(n) >= 0; }
 
-int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st) {
+// The schedule holding every twiddle that a compact-twiddle ("<name>.ct")
+// schedule is derived from, or the schedule itself.
+template <class Entry>
+static int precise_partner(const Entry* list, int count, int sel) {
+  const char* name = list[sel].name;
+  const size_t len = std::strlen(name);
+  if (len < 3 || std::strcmp(name + len - 3, ".ct") != 0) return sel;
+  for (int v = 0; v < count; ++v)
+    if (std::strlen(list[v].name) == len - 3 && std::strncmp(list[v].name, name, len - 3) == 0) return v;
+  return sel;
+}
+
+int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st, bool precise) {
   const int i = size_index(Nx);
   if (i < 0) return fail(kErrUnsupported, "unsupported row length %d", Nx);
   init_selection();
   const SchedTable* t = table_for(Nx);
-  const int sel = g_row_sel[i].load(std::memory_order_relaxed);
+  int sel = g_row_sel[i].load(std::memory_order_relaxed);
+  if (precise) sel = precise_partner(t->rows, t->n_rows, sel);
   return t->rows[sel].launch(op, a, st);
 }
 
-int col_pass(int Ny, int op, ColArgs& a, hipStream_t st) {
+int col_pass(int Ny, int op, ColArgs& a, hipStream_t st, bool precise) {
   const int i = size_index(Ny);
   if (i < 0) return fail(kErrUnsupported, "unsupported column length %d", Ny);
   init_selection();
   const SchedTable* t = table_for(Ny);
-  const int sel = g_col_sel[i].load(std::memory_order_relaxed);
+  int sel = g_col_sel[i].load(std::memory_order_relaxed);
+  if (precise) sel = precise_partner(t->cols, t->n_cols, sel);
   return t->cols[sel].launch(op, a, st);
 }
 

@@ -54,12 +54,17 @@ constexpr int kMinColTile = 4;
 constexpr int kMaxColTile = 16;
 // row steps followed by a band-limited column pass store this-aligned bands
 constexpr int kStoreAlign = kMaxColTile;
+// Thicker runs than this take the slice loop's passes from the schedules that
+// hold every twiddle instead of compact-twiddle ones: the round-off of the
+// compact forms (about 9e-8 of the CBED per slice) would approach 1e-4 by
+// 1000 slices; the full forms accumulate about 4.5e-8 per slice.
+constexpr int kPreciseSlices = 300;
 
 bool supported(int n);
-int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st);
+int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st, bool precise = false);
 // Launches the selected column schedule; on return a.strips_per_image holds
 // the strip count that was used.
-int col_pass(int Ny, int op, ColArgs& a, hipStream_t st);
+int col_pass(int Ny, int op, ColArgs& a, hipStream_t st, bool precise = false);
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
