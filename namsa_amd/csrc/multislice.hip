@@ -309,7 +309,7 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
     // band; strips are at most kStoreAlign wide and aligned to their width,
     // so the row pass stores just the kStoreAlign-aligned band.
     const int last_prop = (mode == NAMSA_OUT_EXIT) ? S : S - 1;
-    const bool precise = S > kPreciseSlices;
+    const bool precise = S > precise_slices(Ny) || S > precise_slices(Nx);
     const int store_lo = band_lo / kStoreAlign * kStoreAlign;
     const int store_hi = std::min(Nx, (band_hi + kStoreAlign - 1) / kStoreAlign * kStoreAlign);
     for (int i = 0; i < S; ++i) {

@@ -34,14 +34,13 @@ static const SchedTable* table_for(int n) {
 
 // Default schedule per length and pass: the fastest schedules measured on
 // MI355X at the cache-sized probe group (3/4 of the 256 MB LLC of waves per
-// launch, timed as back-to-back slice steps).  For 512 and up these are the
-// compact-twiddle (.ct) schedules, 9-11% faster than the ones holding every
-// twiddle, with 1.2-1.35x their round-off (about 6e-5 of the CBED after 1000
-// vacuum slices).  The radix-3 lengths keep every twiddle: with 4-5 stages
-// their round-off is already 3.5e-5 after 300 slices, 6.2e-5 in compact form.
+// launch, timed as back-to-back slice steps).  From 384 up these are the
+// compact-twiddle (.ct) schedules, 9-27% faster than the ones holding every
+// twiddle; the fused driver switches to the latter for thick runs (see
+// precise_slices), where the compact forms' round-off would approach 1e-4.
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 1,  0,  2,  0,  5,  0,  3,  2};
-static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  1, 13,  1,  3,  3};
+static const int kDefaultRow[kNumSizes] = { 1,  0,  2,  2,  5,  2,  3,  2};
+static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  2, 13,  2,  3,  3};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];

@@ -55,10 +55,11 @@ constexpr int kMaxColTile = 16;
 // row steps followed by a band-limited column pass store this-aligned bands
 constexpr int kStoreAlign = kMaxColTile;
 // Thicker runs than this take the slice loop's passes from the schedules that
-// hold every twiddle instead of compact-twiddle ones: the round-off of the
-// compact forms (about 9e-8 of the CBED per slice) would approach 1e-4 by
-// 1000 slices; the full forms accumulate about 4.5e-8 per slice.
-constexpr int kPreciseSlices = 300;
+// hold every twiddle instead of compact-twiddle ones.  The compact forms
+// accumulate about 9e-8 of the CBED per slice for the power-of-two lengths
+// and 2e-7 for the 4-5 stage radix-3 lengths (the full forms 4.5e-8 and
+// 1.2e-7), so the switch keeps every thickness below ~6e-5.
+constexpr int precise_slices(int n) { return n % 3 == 0 ? 200 : 300; }
 
 bool supported(int n);
 int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st, bool precise = false);
