@@ -60,6 +60,12 @@ int namsa_sched_count(int n, int pass);
 const char* namsa_sched_name(int n, int pass, int v);
 int namsa_sched_get(int n, int pass);
 int namsa_sched_set(int n, int pass, int v);
+/* Compact-twiddle (".ct") schedules keep one register set of twiddles per
+ * butterfly and form the rest; namsa_multislice runs for more than
+ * namsa_precise_slices(n) slices use namsa_sched_precise(n, pass, v) -- the
+ * schedule holding every twiddle that v derives from (v itself otherwise). */
+int namsa_precise_slices(int n);
+int namsa_sched_precise(int n, int pass, int v);
 
 /*
  * Kirkland projected-potential tables, one Py x Px float32 table per element

@@ -46,6 +46,24 @@ def selected(n: int, pass_: int) -> int:
     return v
 
 
+def precise(n: int, pass_: int, v=None) -> int:
+    """The schedule the fused driver runs instead of v (default: the selected
+    one) for thick runs: the form holding every twiddle of a compact one."""
+    lib = _lib.load()
+    v = selected(n, pass_) if v is None else v
+    r = lib.namsa_sched_precise(n, pass_, int(v))
+    if r < 0:
+        raise _lib.NamsaError(lib.namsa_last_error().decode())
+    return r
+
+
+def driver_schedules(n: int, slices: int) -> tuple:
+    """(row, column) schedule indices namsa_multislice runs for `slices` slices."""
+    lib = _lib.load()
+    thick = slices > lib.namsa_precise_slices(n)
+    return tuple(precise(n, p) if thick else selected(n, p) for p in (PASS_ROW, PASS_COL))
+
+
 def select(n: int, pass_: int, v) -> None:
     if isinstance(v, str):
         v = schedules(n, pass_).index(v)

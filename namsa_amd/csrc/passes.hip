@@ -118,6 +118,15 @@ int namsa_sched_get(int n, int pass) {
   return pass == PASS_ROW ? g_row_sel[i].load() : g_col_sel[i].load();
 }
 
+int namsa_precise_slices(int n) { return supported(n) ? precise_slices(n) : -1; }
+
+int namsa_sched_precise(int n, int pass, int v) {
+  const SchedTable* t = table_for(n);
+  if (!t || v < 0 || v >= namsa_sched_count(n, pass))
+    return fail(kErrArg, "namsa_sched_precise: bad length %d, pass %d or schedule %d", n, pass, v);
+  return pass == PASS_ROW ? precise_partner(t->rows, t->n_rows, v) : precise_partner(t->cols, t->n_cols, v);
+}
+
 int namsa_sched_set(int n, int pass, int v) {
   const int i = size_index(n);
   if (i < 0 || v < 0 || v >= namsa_sched_count(n, pass))

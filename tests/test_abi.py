@@ -71,3 +71,22 @@ def test_schedule_registry_host_side():
     assert lib.namsa_sched_count(500, 0) == 0
     assert lib.namsa_sched_name(512, 0, 99) is None
     assert lib.namsa_sched_set(512, 1, 99) == -1
+
+
+def test_precise_schedule_policy():
+    """Thick runs swap each compact-twiddle schedule for the schedule holding
+    every twiddle it derives from; other schedules map to themselves."""
+    from namsa_amd import autotune
+
+    lib = L.load()
+    assert lib.namsa_precise_slices(512) == 300 and lib.namsa_precise_slices(768) == 200
+    assert lib.namsa_precise_slices(500) == -1
+    for n in (128, 192, 256, 384, 512, 768, 1024, 2048):
+        for p in (0, 1):
+            names = autotune.schedules(n, p)
+            for v, name in enumerate(names):
+                q = lib.namsa_sched_precise(n, p, v)
+                expect = names.index(name[:-3]) if name.endswith(".ct") else v
+                assert q == expect, (n, p, name, names[q])
+    assert lib.namsa_sched_precise(512, 0, 99) < 0
+    assert autotune.driver_schedules(512, 150) == (autotune.selected(512, 0), autotune.selected(512, 1))
