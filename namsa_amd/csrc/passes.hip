@@ -59,7 +59,8 @@ static void init_selection() {
 bool supported(int n) { return size_index(n) >= 0; }
 
 // The schedule holding every twiddle that a compact-twiddle ("<name>.ct")
-// schedule is derived from, or the schedule itself.
+// schedule is derived from (the base schedule 0 if it has no such partner),
+// or the schedule itself.
 template <class Entry>
 static int precise_partner(const Entry* list, int count, int sel) {
   const char* name = list[sel].name;
@@ -67,7 +68,7 @@ static int precise_partner(const Entry* list, int count, int sel) {
   if (len < 3 || std::strcmp(name + len - 3, ".ct") != 0) return sel;
   for (int v = 0; v < count; ++v)
     if (std::strlen(list[v].name) == len - 3 && std::strncmp(list[v].name, name, len - 3) == 0) return v;
-  return sel;
+  return 0;
 }
 
 int row_pass(int Nx, int op, const RowArgs& a, hipStream_t st, bool precise) {

@@ -47,14 +47,12 @@ using C0 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 16, false, false>;
 using C1 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false>;
 using R2 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, false, false, true>;
 using C2 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, false, true>;
-using C3 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 16, true, false, false, true>;
 static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>},
                                       {"e12.r32444.pf", launch_row<R1, R0>},
                                       {"e12.r32444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r32444.c16", launch_col<C0, C0>, 16},
                                       {"e12.r32444.c8.pf", launch_col<C1, C0>, 8},
-                                      {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8},
-                                      {"e12.r32444.c16.pf.ct", launch_col<C3, C0>, 16}};
+                                      {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8}};
 #elif NAMSA_PASS_N == 512
 using R0 = RowS<512, 8, Radices<8, 8, 8>, false, false>;
 using R1 = RowS<512, 8, Radices<8, 8, 8>, true, false>;
@@ -103,14 +101,12 @@ using C0 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false>;
 using C1 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false>;
 using R2 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false, true>;
 using C2 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, false, true>;
-using C4 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 16, true, false, false, true>;
 static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>},
                                       {"e12.r34444.pf", launch_row<R1, R0>},
                                       {"e12.r34444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
                                       {"e12.r34444.c8.pf", launch_col<C1, C0>, 8},
-                                      {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8},
-                                      {"e12.r34444.c16.pf.ct", launch_col<C4, C0>, 16}};
+                                      {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8}};
 #elif NAMSA_PASS_N == 1024
 using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
@@ -124,12 +120,10 @@ static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
                                       {"e8.r8844.pf", launch_row<R2, R0>},
                                       {"e16.r16.8.8.ct", launch_row<R3, R0>}};
 using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
-using C4 = ColS<1024, 16, Radices<16, 8, 8>, 16, true, false, false, true>;
 static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
                                       {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
                                       {"e8.r8844.c8", launch_col<C2, C0>, 8},
-                                      {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8},
-                                      {"e16.r16.8.8.c16.pf.ct", launch_col<C4, C0>, 16}};
+                                      {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8}};
 #elif NAMSA_PASS_N == 2048
 using R0 = RowS<2048, 16, Radices<16, 16, 8>, false, false>;
 using R1 = RowS<2048, 16, Radices<16, 16, 8>, true, false>;
@@ -138,15 +132,13 @@ using C1 = ColS<2048, 16, Radices<16, 16, 8>, 8, false, false>;
 using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
 using R2 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true>;
 using C3 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false, false, true>;
-using C4 = ColS<2048, 16, Radices<16, 16, 8>, 8, true, false, false, true>;
 static const RowSchedEntry kRows[] = {{"e16.r16.16.8", launch_row<R0, R0>},
                                       {"e16.r16.16.8.pf", launch_row<R1, R0>},
                                       {"e16.r16.16.8.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r16.16.8.c4", launch_col<C0, C0>, 4},
                                       {"e16.r16.16.8.c8", launch_col<C1, C0>, 8},
                                       {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4},
-                                      {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4},
-                                      {"e16.r16.16.8.c8.pf.ct", launch_col<C4, C0>, 8}};
+                                      {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4}};
 #else
 #error "no schedules for this transform length"
 #endif
