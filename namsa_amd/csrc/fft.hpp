@@ -230,34 +230,101 @@ struct ColLds {
   static constexpr int padded(int n) { return n * C + (n * C) / 32; }
 };
 
+// ---------------------------------------------------------------- twiddles
+// Twiddles are fixed by (length, schedule, thread), so each schedule's set is
+// computed at compile time into a table in the code object; a kernel loads a
+// thread's twiddles once (coalesced: entry i of thread t at i * T + t).
+// Computing them at run time (FP64 sincospi per twiddle in every workgroup)
+// cost about 3 us per launch of the slice loop's passes.
+namespace twc {
+
+struct Pair {
+  float c, s;
+};
+
+constexpr double kTwoPi = 6.283185307179586476925286766559005768;
+
+// sin x and cos x for |x| <= pi/4 from their Taylor series through x^29 and
+// x^28 (truncation below 1e-22), summed from the smallest term.
+constexpr void sincos_small(double x, double& s, double& c) {
+  constexpr int K = 15;
+  const double x2 = x * x;
+  double ts[K] = {}, tc[K] = {};
+  ts[0] = x;
+  tc[0] = 1.0;
+  for (int n = 1; n < K; ++n) {
+    ts[n] = ts[n - 1] * (-x2 / double((2 * n) * (2 * n + 1)));
+    tc[n] = tc[n - 1] * (-x2 / double((2 * n - 1) * (2 * n)));
+  }
+  s = 0.0;
+  c = 0.0;
+  for (int n = K - 1; n >= 0; --n) {
+    s += ts[n];
+    c += tc[n];
+  }
+}
+
+constexpr double abs_d(double x) { return x < 0 ? -x : x; }
+
+// nextafterf(x, up ? +inf : -inf)
+constexpr float next_float(float x, bool up) {
+  if (x == 0.0f) return up ? 1.40129846e-45f : -1.40129846e-45f;
+  unsigned b = __builtin_bit_cast(unsigned, x);
+  b = ((x > 0.0f) == up) ? b + 1u : b - 1u;
+  return __builtin_bit_cast(float, b);
+}
+
 // exp(i theta) as a float pair whose magnitude is as close to 1 as float
 // allows: of the two float neighbours of each of cos and sin, the pair with
 // the smallest |c^2 + s^2 - 1|.  Rounding cos and sin independently leaves a
 // magnitude error of a few 1e-8 that does not average out over a transform's
 // few distinct twiddles, and it compounds over hundreds of slices (and over
 // the powers of the compact-twiddle mode).
-__device__ __forceinline__ float2 unit_twiddle(double c, double s) {
+constexpr Pair unit_pair(double c, double s) {
   const float c0 = float(c), s0 = float(s);
-  const float c1 = double(c0) > c ? nextafterf(c0, -2.0f) : nextafterf(c0, 2.0f);
-  const float s1 = double(s0) > s ? nextafterf(s0, -2.0f) : nextafterf(s0, 2.0f);
-  float2 best = make_float2(c0, s0);
-  double err = fabs(double(c0) * c0 + double(s0) * s0 - 1.0);
+  const float c1 = double(c0) > c ? next_float(c0, false) : next_float(c0, true);
+  const float s1 = double(s0) > s ? next_float(s0, false) : next_float(s0, true);
   const float cs[2] = {c0, c1}, ss[2] = {s0, s1};
-#pragma unroll
+  Pair best{c0, s0};
+  double err = abs_d(double(c0) * c0 + double(s0) * s0 - 1.0);
   for (int i = 0; i < 2; ++i)
-#pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const double e = fabs(double(cs[i]) * cs[i] + double(ss[j]) * ss[j] - 1.0);
+      const double e = abs_d(double(cs[i]) * cs[i] + double(ss[j]) * ss[j] - 1.0);
       if (e < err) {
         err = e;
-        best = make_float2(cs[i], ss[j]);
+        best = Pair{cs[i], ss[j]};
       }
     }
   return best;
 }
 
+// exp(2 pi i m / d): the angle is split exactly into the nearest quarter
+// turn q and a remainder of at most 1/8 turn, so the series only sees
+// |x| <= pi/4 and the quarter-turn rotation is exact.
+constexpr Pair unit_turns(long long m, long long d) {
+  m %= d;
+  if (m < 0) m += d;
+  const long long q = (4 * m + d / 2) / d;
+  const double x = kTwoPi * double(4 * m - q * d) / double(4 * d);
+  double s = 0.0, c = 0.0;
+  sincos_small(x, s, c);
+  switch (q & 3) {
+    case 0: return unit_pair(c, s);
+    case 1: return unit_pair(-s, c);
+    case 2: return unit_pair(-c, -s);
+    default: return unit_pair(s, -c);
+  }
+}
+
+}  // namespace twc
+
 template <int N, int E, class RL, bool CT = false>
 struct Fft;
+
+// A schedule's twiddle table (Fft::make_table), evaluated by the compiler and
+// stored in the code object's read-only data.
+template <class F>
+__device__ constexpr typename F::Table kTwTable = F::make_table();
 
 template <int N, int E, bool CT, int... Rs>
 struct Fft<N, E, Radices<Rs...>, CT> {
@@ -274,32 +341,46 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   static constexpr int NSTAGES = sizeof...(Rs);
 
   // ---------------- twiddles (forward sign; inverse uses the conjugate)
+  // Entry OFF + q * P + (r - 1) of thread t is w^p with w = exp(-2 pi i k /
+  // (NS R)), k = (t + T q) mod NS, p = stored_power(r): the r-th stored power
+  // for butterfly q of the stage whose sub-transforms have length NS.
+  struct Table {
+    twc::Pair v[NTW_ALLOC * T];
+  };
+
   template <int NS, int OFF>
-  static __device__ __forceinline__ void tw_stages(int, float2 (&)[NTW_ALLOC]) {}
+  static constexpr void fill_stages(Table&) {}
 
   template <int NS, int OFF, int R, int... Rest>
-  static __device__ __forceinline__ void tw_stages(int t, float2 (&tw)[NTW_ALLOC]) {
+  static constexpr void fill_stages(Table& tab) {
     constexpr int Q = E / R;
     constexpr int P = per_bfly<R>();
     if constexpr (NS > 1) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int k = (t + T * q) % NS;
-#pragma unroll
-        for (int r = 1; r <= P; ++r) {
-          double s, c;
-          sincospi(-2.0 * double(stored_power<R>(r) * k) / double(NS * R), &s, &c);
-          tw[OFF + q * P + (r - 1)] = unit_twiddle(c, s);
+      for (int t = 0; t < T; ++t)
+        for (int q = 0; q < Q; ++q) {
+          const int k = (t + T * q) % NS;
+          for (int r = 1; r <= P; ++r)
+            tab.v[(OFF + q * P + (r - 1)) * T + t] =
+                twc::unit_turns(-static_cast<long long>(stored_power<R>(r)) * k, static_cast<long long>(NS) * R);
         }
-      }
-      tw_stages<NS * R, OFF + Q * P, Rest...>(t, tw);
+      fill_stages<NS * R, OFF + Q * P, Rest...>(tab);
     } else {
-      tw_stages<NS * R, OFF, Rest...>(t, tw);
+      fill_stages<NS * R, OFF, Rest...>(tab);
     }
   }
 
+  static constexpr Table make_table() {
+    Table tab{};
+    fill_stages<1, 0, Rs...>(tab);
+    return tab;
+  }
+
   static __device__ __forceinline__ void twiddles(int t, float2 (&tw)[NTW_ALLOC]) {
-    tw_stages<1, 0, Rs...>(t, tw);
+    if constexpr (NTW > 0) {
+      const twc::Pair* p = kTwTable<Fft>.v + t;
+#pragma unroll
+      for (int i = 0; i < NTW; ++i) tw[i] = make_float2(p[i * T].c, p[i * T].s);
+    }
   }
 
   // ---------------- one stage
