@@ -114,7 +114,9 @@ def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbo
     import torch
 
     if batch <= 0:
-        batch = max(1, (256 << 20) * 3 // 4 // (8 * n * n))
+        from .msa import cache_group
+
+        batch = cache_group(n, n)
 
     dev = torch.device("cuda", torch.cuda.current_device())
     rng = np.random.default_rng(0)

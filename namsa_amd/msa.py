@@ -49,8 +49,17 @@ def _torch():
     return torch
 
 
+# Share of the last-level cache that one probe group's waves should fill, per
+# square grid: the fastest of 0.5-1.5x measured on MI355X for the slice loop
+# (20 slices, bandwidth 1/3).  Larger groups amortise the fixed cost of each
+# pass launch; past the cache the wave traffic spills to HBM.  At 2048^2 the
+# slice and propagator planes (64 MB) also compete for the cache.
+_GROUP_SHARE = {128: 0.75, 192: 1.0, 256: 0.875, 384: 1.0, 512: 1.0, 768: 1.0, 1024: 1.0, 2048: 0.75}
+
+
 def cache_group(Ny, Nx):
-    """Probes whose complex64 waves fill ~3/4 of the last-level cache.
+    """Probes per group: complex64 waves filling the measured best share of
+    the last-level cache (3/4 for grids without a measurement).
 
     Pushing such a group through all slices before starting the next keeps the
     slice loop's wave traffic in the cache instead of HBM.  The autotuner's
@@ -61,7 +70,8 @@ def cache_group(Ny, Nx):
     g = autotune.cached_group(Ny, Nx)
     if g:
         return g
-    return max(1, int(LLC_BYTES * 3 // 4 // (8 * Ny * Nx)))
+    share = _GROUP_SHARE.get(Nx, 0.75) if Ny == Nx else 0.75
+    return max(1, int(LLC_BYTES * share // (8 * Ny * Nx)))
 
 
 class MSAGPU:
