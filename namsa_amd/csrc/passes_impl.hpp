@@ -97,6 +97,16 @@ __global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
     const long long r = rb * G + g;
     return r < a.n_rows ? r : a.n_rows - 1;
   };
+  // Row of the slice for the next row loaded (row mod rows_per_image), kept
+  // incrementally: a 64-bit modulo per row costs ~40 VALU instructions.  Only
+  // a clamped tail row (never stored) can get another in-range slice row.
+  const int rows_img = a.rows_per_image;
+  const int row_step = static_cast<int>((static_cast<long long>(gridDim.x) * G) % rows_img);
+  int trow = static_cast<int>(clamp_row(blockIdx.x) % rows_img);
+  auto next_trow = [&]() {
+    trow += row_step;
+    if (trow >= rows_img) trow -= rows_img;
+  };
   auto load = [&](long long r, float2(&v)[E], float2(&tv)[E]) {
     const float2* p = a.psi + r * N + t;
 #pragma unroll
@@ -105,7 +115,7 @@ __global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
       v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + T * m) : make_float2(0.f, 0.f);
     }
     if constexpr (STEP) {
-      const float2* tr = a.tsl + static_cast<long long>(r % a.rows_per_image) * N + t;
+      const float2* tr = a.tsl + trow * N + t;
 #pragma unroll
       for (int m = 0; m < E; ++m) tv[m] = tr[T * m];
     }
@@ -126,9 +136,11 @@ __global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
         v[m] = pv[m];
         tv[m] = pt[m];
       }
+      next_trow();
       if (rb + gridDim.x < n_blocks) load(clamp_row(rb + gridDim.x), pv, pt);
     } else {
       load(rr, v, tv);
+      next_trow();
     }
     if constexpr (STEP) {
       F::template run<true, Sync>(v, lds, t, tw);
