@@ -33,14 +33,15 @@ static const SchedTable* table_for(int n) {
 }
 
 // Default schedule per length and pass: the fastest schedules measured on
-// MI355X at the cache-sized probe group (3/4 of the 256 MB LLC of waves per
-// launch, timed as back-to-back slice steps).  From 384 up these are the
-// compact-twiddle (.ct) schedules, 9-27% faster than the ones holding every
-// twiddle; the fused driver switches to the latter for thick runs (see
-// precise_slices), where the compact forms' round-off would approach 1e-4.
+// MI355X at the production probe group (timed as back-to-back slice steps).
+// From 384 up the row pass uses the compact-twiddle (.ct) schedule, 2-3%
+// faster than the one holding every twiddle; the fused driver switches to the
+// latter for thick runs (see precise_slices), where the compact form's
+// round-off would approach 1e-4.  The column pass holds every twiddle: with
+// the twiddles loaded from tables its compact forms are 1-2% slower.
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 1,  0,  2,  2,  5,  2,  3,  2};
-static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  2, 13,  2,  3,  3};
+static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  5,  2,  3,  2};
+static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  1, 10,  1,  1,  2};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];

@@ -165,6 +165,25 @@ def test_cbed_parity(gpu, N, S_thick):
     assert max(errs) < 1e-4
 
 
+@pytest.mark.parametrize("Ny,Nx", [(256, 512), (384, 256)])
+def test_cbed_parity_rectangular(gpu, Ny, Nx):
+    # sampling = [Ny, Nx] with Ny != Nx: per-axis pixel sizes, one scalar kmax
+    # (msa.py:114,121-122), the band on the x columns, different row and
+    # column transform lengths
+    _say(f"cbed parity rectangular Ny={Ny} Nx={Nx}")
+    rec = random_block(7 + Ny + Nx, 24.0, 12.0, int(24 * 24 * 12.0 * 0.06))
+    sim = MSAGPU(200, 0.025, rec, sampling=np.array([Ny, Nx]), debye_waller=False)
+    pos = np.array([[0.0, 0.0], [3.3, -2.1], [-5.25, 7.7]])
+    _run_sim(sim, 2.0, pos)
+    t_ref, _ = oracle_slices(sim, 2.0)
+    psi_k, _ = oracle_probe(sim)
+    cb_ref = O.cbed(psi_k, pos.astype(np.float32), np.float32(sim.kmax), t_ref, oracle_pm(sim, 2.0, 1 / 3))
+    assert sim.probes.shape == (len(pos), Ny, Nx)
+    errs = [rel_l2(sim.probes[i], cb_ref[i]) for i in range(len(pos))]
+    _say(f"  S={sim.num_slices} CBED relL2 per probe {['%.2e' % e for e in errs]}")
+    assert max(errs) < 1e-4
+
+
 def test_exit_wave_and_incident_parity(gpu):
     _say("exit wave / incident 256^2")
     rec = random_block(7, 30.0, 16.0, 800)
