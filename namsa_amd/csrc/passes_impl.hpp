@@ -51,9 +51,12 @@ __device__ __forceinline__ void st(float2* p, float2 v) {
   }
 }
 
-template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false>
+// W > 0 asks the compiler for at least W wavefronts per SIMD (a register
+// budget of 512 / W VGPRs): the row pass keeps most of its memory traffic
+// under its arithmetic only with enough wavefronts in flight.
+template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false, int W_ = 0>
 struct RowS {
-  static constexpr int N = N_, E = E_;
+  static constexpr int N = N_, E = E_, W = W_;
   using RL = RL_;
   static constexpr bool PF = PF_, NT = NT_, CT = CT_;
 };
@@ -77,7 +80,7 @@ __device__ __forceinline__ float2 unit_phase(double turns) {
 
 // ------------------------------------------------------------------ row pass
 template <class S, int OP>
-__global__ void __launch_bounds__(kRowThreads) row_kernel(RowArgs a) {
+__global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(RowArgs a) {
   constexpr int N = S::N, E = S::E;
   using F = Fft<N, E, typename S::RL, S::CT>;
   constexpr int T = F::T;

@@ -2,7 +2,8 @@
 // Built once per length with -DNAMSA_PASS_N=<n>; entry 0 is the base schedule.
 // Names read: e<E> = elements per thread, r<radices>, c<C> = column tile,
 // pf = next row/tile prefetched, nt = non-temporal wave traffic, ct = compact
-// twiddles (one register pair per butterfly, higher powers formed in the stage).
+// twiddles (one register pair per butterfly, higher powers formed in the stage),
+// w<k> = register budget for at least k wavefronts per SIMD.
 #include "passes_impl.hpp"
 
 #ifndef NAMSA_PASS_N
@@ -16,7 +17,10 @@ using R0 = RowS<128, 8, Radices<8, 4, 4>, false, false>;
 using R1 = RowS<128, 8, Radices<8, 4, 4>, true, false>;
 using C0 = ColS<128, 8, Radices<8, 4, 4>, 16, false, false>;
 using C1 = ColS<128, 8, Radices<8, 4, 4>, 8, true, false>;
-static const RowSchedEntry kRows[] = {{"e8.r844", launch_row<R0, R0>}, {"e8.r844.pf", launch_row<R1, R0>}};
+using R2 = RowS<128, 8, Radices<8, 4, 4>, false, false, false, 6>;
+static const RowSchedEntry kRows[] = {{"e8.r844", launch_row<R0, R0>},
+                                      {"e8.r844.pf", launch_row<R1, R0>},
+                                      {"e8.r844.w6", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e8.r844.c16", launch_col<C0, C0>, 16},
                                       {"e8.r844.c8.pf", launch_col<C1, C0>, 8}};
 #elif NAMSA_PASS_N == 192
@@ -31,12 +35,14 @@ static const ColSchedEntry kCols[] = {{"e12.r3444.c16", launch_col<C0, C0>, 16},
 using R0 = RowS<256, 8, Radices<8, 8, 4>, false, false>;
 using R1 = RowS<256, 8, Radices<8, 8, 4>, true, false>;
 using R2 = RowS<256, 16, Radices<16, 16>, true, false>;
+using R3 = RowS<256, 8, Radices<8, 8, 4>, false, false, false, 5>;
 using C0 = ColS<256, 8, Radices<8, 8, 4>, 16, false, false>;
 using C1 = ColS<256, 8, Radices<8, 8, 4>, 8, true, false>;
 using C2 = ColS<256, 16, Radices<16, 16>, 16, true, false>;
 static const RowSchedEntry kRows[] = {{"e8.r884", launch_row<R0, R0>},
                                       {"e8.r884.pf", launch_row<R1, R0>},
-                                      {"e16.r16.16.pf", launch_row<R2, R0>}};
+                                      {"e16.r16.16.pf", launch_row<R2, R0>},
+                                      {"e8.r884.w5", launch_row<R3, R0>}};
 static const ColSchedEntry kCols[] = {{"e8.r884.c16", launch_col<C0, C0>, 16},
                                       {"e8.r884.c8.pf", launch_col<C1, C0>, 8},
                                       {"e16.r16.16.c16.pf", launch_col<C2, C0>, 16}};
@@ -74,12 +80,14 @@ using C11 = ColS<512, 16, Radices<8, 8, 8>, 16, true, false, true>;
 using C12 = ColS<512, 16, Radices<16, 16, 2>, 16, true, false>;
 using C13 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, false, true>;
 using R5 = RowS<512, 8, Radices<8, 8, 8>, false, false, true>;
+using R6 = RowS<512, 8, Radices<8, 8, 8>, false, false, false, 5>;
 static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
                                       {"e8.r888.pf", launch_row<R1, R0>},
                                       {"e8.r888.pf.nt", launch_row<R2, R0>},
                                       {"e16.r16.16.2", launch_row<R3, R0>},
                                       {"e16.r16.16.2.pf", launch_row<R4, R0>},
-                                      {"e8.r888.ct", launch_row<R5, R0>}};
+                                      {"e8.r888.ct", launch_row<R5, R0>},
+                                      {"e8.r888.w5", launch_row<R6, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e8.r888.c8", launch_col<C1, C0>, 8},
                                       {"e8.r888.c8.pf", launch_col<C2, C0>, 8},
@@ -112,13 +120,15 @@ using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
 using R2 = RowS<1024, 8, Radices<8, 8, 4, 4>, true, false>;
 using R3 = RowS<1024, 16, Radices<16, 8, 8>, false, false, true>;
+using R4 = RowS<1024, 16, Radices<16, 8, 8>, false, false, false, 3>;
 using C0 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false>;
 using C1 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false>;
 using C2 = ColS<1024, 8, Radices<8, 8, 4, 4>, 8, false, false>;
 static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
                                       {"e16.r16.8.8.pf", launch_row<R1, R0>},
                                       {"e8.r8844.pf", launch_row<R2, R0>},
-                                      {"e16.r16.8.8.ct", launch_row<R3, R0>}};
+                                      {"e16.r16.8.8.ct", launch_row<R3, R0>},
+                                      {"e16.r16.8.8.w3", launch_row<R4, R0>}};
 using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
 static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
                                       {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
