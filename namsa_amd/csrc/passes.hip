@@ -35,16 +35,16 @@ static const SchedTable* table_for(int n) {
 // Default schedule per length and pass: the fastest schedules measured on
 // MI355X at the production probe group (timed as back-to-back slice steps).
 // The row pass keeps its memory traffic under its arithmetic only with enough
-// wavefronts per SIMD: at 512^2 and 1024^2 the schedules holding every
-// twiddle reach that under a register budget (.w5, .w3); at 384^2, 768^2 and
-// 2048^2 only the compact-twiddle (.ct) ones do (2-3% faster than the full
-// ones), and the fused driver switches those to the full ones for thick runs
-// (see precise_slices), where the compact forms' round-off would approach
-// 1e-4.  The column pass holds every twiddle: with twiddles loaded from
-// tables its compact forms are 1-2% slower.
+// wavefronts per SIMD: at 512^2 the schedule holding every twiddle reaches
+// that under a register budget (.w5); from 384^2 up elsewhere only the
+// compact-twiddle (.ct) ones do (2-3% faster than the full ones), and the
+// fused driver switches those to the full ones for thick runs (see
+// precise_slices), where the compact forms' round-off would approach 1e-4.
+// The column pass holds every twiddle: with twiddles loaded from tables its
+// compact forms are 1-2% slower.
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 0,  0,  0,  2,  6,  2,  4,  2};
-static const int kDefaultCol[kNumSizes] = { 1,  0,  2,  1, 10,  1,  1,  2};
+static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  6,  2,  3,  2};
+static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  1, 10,  1,  1,  2};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];
