@@ -40,11 +40,14 @@ static const SchedTable* table_for(int n) {
 // compact-twiddle (.ct) ones do (2-3% faster than the full ones), and the
 // fused driver switches those to the full ones for thick runs (see
 // precise_slices), where the compact forms' round-off would approach 1e-4.
-// The column pass holds every twiddle: with twiddles loaded from tables its
-// compact forms are 1-2% slower.
+// The column pass double-buffers its LDS exchanges (one barrier per
+// exchange instead of two) at 384^2-768^2 and holds every twiddle (its
+// compact forms are 1-2% slower with twiddles loaded from tables), except at
+// 512^2, where only the compact form makes room for the second buffer
+// without spilling.
 //                                       128 192 256 384 512 768 1024 2048
 static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  6,  2,  3,  2};
-static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  1, 10,  1,  1,  2};
+static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  3, 14,  3,  1,  2};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];

@@ -53,12 +53,14 @@ using C0 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 16, false, false>;
 using C1 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false>;
 using R2 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, false, false, true>;
 using C2 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, false, true>;
+using C3 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, true>;
 static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>},
                                       {"e12.r32444.pf", launch_row<R1, R0>},
                                       {"e12.r32444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r32444.c16", launch_col<C0, C0>, 16},
                                       {"e12.r32444.c8.pf", launch_col<C1, C0>, 8},
-                                      {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8}};
+                                      {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8},
+                                      {"e12.r32444.c8.db.pf", launch_col<C3, C0>, 8}};
 #elif NAMSA_PASS_N == 512
 using R0 = RowS<512, 8, Radices<8, 8, 8>, false, false>;
 using R1 = RowS<512, 8, Radices<8, 8, 8>, true, false>;
@@ -79,6 +81,7 @@ using C10 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false>;
 using C11 = ColS<512, 16, Radices<8, 8, 8>, 16, true, false, true>;
 using C12 = ColS<512, 16, Radices<16, 16, 2>, 16, true, false>;
 using C13 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, false, true>;
+using C14 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, true, true>;
 using R5 = RowS<512, 8, Radices<8, 8, 8>, false, false, true>;
 using R6 = RowS<512, 8, Radices<8, 8, 8>, false, false, false, 5>;
 static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
@@ -101,7 +104,8 @@ static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e8.r888.c16.pf", launch_col<C10, C0>, 16},
                                       {"e16.r888.c16.db.pf", launch_col<C11, C0>, 16},
                                       {"e16.r16.16.2.c16.pf", launch_col<C12, C0>, 16},
-                                      {"e8.r888.c16.pf.ct", launch_col<C13, C0>, 16}};
+                                      {"e8.r888.c16.pf.ct", launch_col<C13, C0>, 16},
+                                      {"e8.r888.c16.db.pf.ct", launch_col<C14, C0>, 16}};
 #elif NAMSA_PASS_N == 768
 using R0 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false>;
 using R1 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, true, false>;
@@ -109,12 +113,14 @@ using C0 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false>;
 using C1 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false>;
 using R2 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false, true>;
 using C2 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, false, true>;
+using C3 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, true>;
 static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>},
                                       {"e12.r34444.pf", launch_row<R1, R0>},
                                       {"e12.r34444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
                                       {"e12.r34444.c8.pf", launch_col<C1, C0>, 8},
-                                      {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8}};
+                                      {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8},
+                                      {"e12.r34444.c8.db.pf", launch_col<C3, C0>, 8}};
 #elif NAMSA_PASS_N == 1024
 using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
