@@ -51,10 +51,11 @@ def _torch():
 
 # Share of the last-level cache that one probe group's waves should fill, per
 # square grid: the fastest of 0.5-1.5x measured on MI355X for the slice loop
-# (20 slices, bandwidth 1/3).  Larger groups amortise the fixed cost of each
-# pass launch; past the cache the wave traffic spills to HBM.  At 2048^2 the
-# slice and propagator planes (64 MB) also compete for the cache.
-_GROUP_SHARE = {128: 0.75, 192: 1.0, 256: 0.875, 384: 1.0, 512: 1.0, 768: 1.0, 1024: 1.0, 2048: 0.75}
+# (bandwidth 1/3).  Larger groups amortise the fixed cost of each pass
+# launch; past the cache the wave traffic spills to HBM.  At 512^2 the optimum
+# (160 probes) lies past the cache and 176 is 6% slower; at 2048^2 the slice
+# and propagator planes (64 MB) also compete for the cache.
+_GROUP_SHARE = {128: 0.75, 192: 1.0, 256: 0.875, 384: 1.0, 512: 1.25, 768: 1.0, 1024: 1.0, 2048: 0.75}
 
 
 def cache_group(Ny, Nx):
