@@ -134,3 +134,17 @@ def test_product_does_not_import_oracle():
     for f in pathlib.Path("namsa_amd").rglob("*.py"):
         src = f.read_text()
         assert "import oracle" not in src and "from oracle" not in src, f
+
+
+def test_probe_group_policy(monkeypatch, tmp_path):
+    # waves of one group fill the measured share of the 256 MB last-level
+    # cache (no tuner cache present); grids without a measurement use 3/4
+    from namsa_amd import msa
+
+    monkeypatch.setenv("NAMSA_TUNE_CACHE", str(tmp_path / "none.json"))
+    llc = 256 << 20
+    for n, share in msa._GROUP_SHARE.items():
+        assert msa.cache_group(n, n) == int(llc * share // (8 * n * n))
+    assert msa.cache_group(512, 512) == 160 and msa.cache_group(2048, 2048) == 6
+    assert msa.cache_group(256, 512) == int(llc * 0.75 // (8 * 256 * 512))
+    assert msa.cache_group(8192, 8192) == 1
