@@ -203,9 +203,9 @@ int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int ba
  *   det_ws:  namsa_det_workspace_doubles(B, Ny, Nx, nDet) float64 scratch values
  *   det_out: [B][nDet] float64
  * The passes run the schedules namsa_sched_set selected; for S > 300 (S > 200
- * for lengths divisible by 3) the slice loop uses the schedules holding every
- * twiddle in place of compact-twiddle (".ct") ones, whose round-off would
- * approach 1e-4 of the CBED by 1000 slices.
+ * for lengths divisible by 3) the slice loop and the readout use the schedules
+ * holding every twiddle in place of compact-twiddle (".ct") ones, whose
+ * round-off would approach 1e-4 of the CBED by 1000 slices.
  */
 int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
                      const void* t, int S, const void* pm, int band_lo, int band_hi, int group,

@@ -337,7 +337,7 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
     a.det = det;
     a.n_det = nDet > 0 ? nDet : 0;
     a.det_ws = det_ws;
-    if (int rc = col_pass(Ny, COL_READOUT, a, st)) return rc;
+    if (int rc = col_pass(Ny, COL_READOUT, a, st, precise)) return rc;
     if (nDet > 0) {
       const int n = Bg * nDet;
       hipLaunchKernelGGL(det_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, st, det_ws,

@@ -274,10 +274,11 @@ def test_vacuum_round_off_growth(gpu, N, S, bound):
     CBED after S slices equals the incident one; the difference is the
     accumulated round-off of the schedules the driver runs.  It grows
     linearly.  Up to 300 slices (200 for the radix-3 lengths 384, 768) the
-    compact-twiddle schedules run: about 9e-8 per slice (2e-7 for radix 3);
-    beyond, with every twiddle held, about 4.5e-8 per slice for the 2-3
-    stage power-of-two transforms (1000 slices stay well inside the 1e-4
-    budget) and 1.2e-7 for the 4-5 stage radix-3 ones."""
+    default schedules run, compact twiddles in some passes (at most about
+    9e-8 per slice, 2e-7 for radix 3, when both passes are compact); beyond,
+    with every twiddle held, about 4.5e-8 per slice for the 2-3 stage
+    power-of-two transforms (1000 slices stay well inside the 1e-4 budget)
+    and 1.2e-7 for the 4-5 stage radix-3 ones."""
     import torch
 
     from namsa_amd._lib import call, ptr, stream_handle
@@ -305,6 +306,55 @@ def test_vacuum_round_off_growth(gpu, N, S, bound):
     err = max(rel_l2(c[1, b], c[0, b]) for b in range(B))
     _say(f"  CBED change after {S} vacuum slices: {err:.2e}")
     assert err < bound
+
+
+@pytest.mark.parametrize("N", [384, 512])
+def test_thick_runs_use_the_exact_partner_schedules(gpu, N):
+    """The fused driver's schedule switch, pinned bit for bit: at the
+    threshold and one slice beyond it, a run with the default selection
+    equals a run with the schedules autotune.driver_schedules names for that
+    thickness selected explicitly."""
+    import torch
+
+    from namsa_amd import autotune
+    from namsa_amd._lib import call, load, ptr, stream_handle
+
+    lib = load()
+    thr = lib.namsa_precise_slices(N)
+    _say(f"thick-run schedule switch N={N} at {thr} and {thr + 1} slices")
+    rng = np.random.default_rng(N)
+    S = thr + 1
+    st = stream_handle()
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex64)).cuda()
+    psi_k = c64((rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N)
+    t = c64(np.exp(1j * rng.uniform(0, 0.05, (S, N, N))))
+    pm = c64(np.exp(1j * rng.uniform(0, 2 * np.pi, (N, N))))
+    pos = torch.tensor([[0.0, 0.0], [2.5, -4.0]], dtype=torch.float32, device="cuda")
+    B = 2
+    ws = torch.empty((B, N, N), dtype=torch.complex64, device="cuda")
+
+    def run(s):
+        cb = torch.empty((B, N, N), device="cuda")
+        call("namsa_multislice", ptr(psi_k), ptr(pos), B, N, N, np.float32(6.0), ptr(t), s, ptr(pm), 0, N, B,
+             L.OUT_CBED, np.float32(1.0), ptr(ws), ptr(cb), None, 0, None, None, st)
+        return cb.cpu().numpy()
+
+    keep = (autotune.selected(N, 0), autotune.selected(N, 1))
+    try:
+        for s in (thr, thr + 1):
+            default = run(s)
+            rows, cols = autotune.driver_schedules(N, s)
+            autotune.select(N, 0, rows)
+            autotune.select(N, 1, cols)
+            explicit = run(s)
+            autotune.select(N, 0, keep[0])
+            autotune.select(N, 1, keep[1])
+            _say(f"  S={s}: {autotune.schedules(N, 0)[rows]} / {autotune.schedules(N, 1)[cols]}")
+            assert np.array_equal(default, explicit)
+        assert autotune.driver_schedules(N, thr) == keep
+    finally:
+        autotune.select(N, 0, keep[0])
+        autotune.select(N, 1, keep[1])
 
 
 def test_polycrystal_2048_parity(gpu):
