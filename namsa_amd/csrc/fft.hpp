@@ -384,7 +384,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   }
 
   // ---------------- one stage
-  template <bool INV, int R, int NS, int OFF, bool LAST, class Lds, class Sync>
+  template <bool INV, int R, int NS, int OFF, bool LAST, class Lds, class Sync, bool LAUNDERED>
   static __device__ __forceinline__ void stage(float2 (&v)[E], Lds& lds, int t,
                                                const float2 (&tw)[NTW_ALLOC]) {
     constexpr int Q = E / R;
@@ -405,8 +405,9 @@ struct Fft<N, E, Radices<Rs...>, CT> {
             float2 x = tw[OFF + q * P + j];
             // opaque to the optimiser (no instruction is emitted): keeps the
             // products from being hoisted out of the persistent loop, which
-            // would bring back one register pair per power
-            asm volatile("" : "+v"(x.x), "+v"(x.y));
+            // would bring back one register pair per power (run() may have
+            // done this once for all stored powers instead)
+            if constexpr (!LAUNDERED) asm volatile("" : "+v"(x.x), "+v"(x.y));
             pw[1 << j] = x;
           }
 #pragma unroll
@@ -418,7 +419,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
           for (int r = 1; r < R; ++r) a[r] = INV ? cmulc(a[r], pw[r]) : cmul(a[r], pw[r]);
         } else if constexpr (CT) {
           float2 w1 = tw[OFF + q];
-          asm volatile("" : "+v"(w1.x), "+v"(w1.y));  // see above: no hoisting
+          if constexpr (!LAUNDERED) asm volatile("" : "+v"(w1.x), "+v"(w1.y));  // see above
           float2 w = w1;
 #pragma unroll
           for (int r = 1; r < R; ++r) {
@@ -456,23 +457,29 @@ struct Fft<N, E, Radices<Rs...>, CT> {
     }
   }
 
-  template <bool INV, int NS, int OFF, class Lds, class Sync>
+  template <bool INV, int NS, int OFF, class Lds, class Sync, bool LAUNDERED>
   static __device__ __forceinline__ void stages(float2 (&)[E], Lds&, int,
                                                 const float2 (&)[NTW_ALLOC]) {}
 
-  template <bool INV, int NS, int OFF, class Lds, class Sync, int R, int... Rest>
+  template <bool INV, int NS, int OFF, class Lds, class Sync, bool LAUNDERED, int R, int... Rest>
   static __device__ __forceinline__ void stages(float2 (&v)[E], Lds& lds, int t,
                                                 const float2 (&tw)[NTW_ALLOC]) {
-    stage<INV, R, NS, OFF, sizeof...(Rest) == 0, Lds, Sync>(v, lds, t, tw);
+    stage<INV, R, NS, OFF, sizeof...(Rest) == 0, Lds, Sync, LAUNDERED>(v, lds, t, tw);
     constexpr int NEXT_OFF = OFF + (NS > 1 ? (E / R) * per_bfly<R>() : 0);
-    stages<INV, NS * R, NEXT_OFF, Lds, Sync, Rest...>(v, lds, t, tw);
+    stages<INV, NS * R, NEXT_OFF, Lds, Sync, LAUNDERED, Rest...>(v, lds, t, tw);
   }
 
-  // Full transform, natural layout in and out.
-  template <bool INV, class Sync, class Lds>
-  static __device__ __forceinline__ void run(float2 (&v)[E], Lds& lds, int t,
-                                             const float2 (&tw)[NTW_ALLOC]) {
-    stages<INV, 1, 0, Lds, Sync, Rs...>(v, lds, t, tw);
+  // Full transform, natural layout in and out.  LAUNDER (compact twiddles
+  // only): pass the stored powers through the empty asm once here, rewriting
+  // the caller's twiddle registers in place, instead of copying each power
+  // before each use in the stages.
+  template <bool INV, class Sync, bool LAUNDER = false, class Lds>
+  static __device__ __forceinline__ void run(float2 (&v)[E], Lds& lds, int t, float2 (&tw)[NTW_ALLOC]) {
+    if constexpr (CT && LAUNDER) {
+#pragma unroll
+      for (int i = 0; i < NTW; ++i) asm volatile("" : "+v"(tw[i].x), "+v"(tw[i].y));
+    }
+    stages<INV, 1, 0, Lds, Sync, CT && LAUNDER, Rs...>(v, lds, t, tw);
   }
 };
 

@@ -262,10 +262,11 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
         load(b, v);
       }
       if constexpr (OP == COL_PROP) {
-        F::template run<false, BlockSync>(v, lds, jt, tw);
+        // compact twiddles: rewritten in place once per transform (fewer moves)
+        F::template run<false, BlockSync, true>(v, lds, jt, tw);
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cst[m]);
-        F::template run<true, BlockSync>(v, lds, jt, tw);
+        F::template run<true, BlockSync, true>(v, lds, jt, tw);
       } else if constexpr (OP == COL_FWD || OP == COL_READOUT) {
         F::template run<false, BlockSync>(v, lds, jt, tw);
       } else if constexpr (OP == COL_INV) {
