@@ -104,6 +104,22 @@ int namsa_build_potential(const float* tables, int nZ, int Py, int Px, const int
                           void* stream);
 
 /*
+ * Slices of a sampled 3D potential, e.g. from an all-electron ab initio
+ * calculation (replaces the trapezoid integration of
+ * MSA.integrate_potential_slices, namsa/msa.py:373-424, and the
+ * exp(1j*sigma*V) of MSAGPU.integrate_potential_slices, msa.py:568-571).
+ *   pot:   [S*d][Ny][Nx] float64 samples, z-major (the caller crops z to S*d
+ *          samples and the plane to the grid, as msa.py:386-397 does)
+ *   z:     [S*d] float64 sample coordinates
+ *   t_out: [S][Ny][Nx] complex64 exp(i sigma V_s), or NULL
+ *   v_out: [S][Ny][Nx] float64 V_s, or NULL (at least one of the two)
+ * V_s = -trapz(pot[s*d .. s*d+d-1], z[s*d .. s*d+d-1]) over its own d samples;
+ * no interval spans two slices, as with the reference's np.split + trapz.
+ */
+int namsa_integrate_slices(const double* pot, const double* z, int S, int d, int Ny, int Nx, double sigma,
+                           void* t_out, double* v_out, void* stream);
+
+/*
  * Probe in k-space psi_k = A(k) exp(-i chi(k)) on the linspace k-grid
  * (replaces soft_aperture / hard_aperture / spherical_phase_error /
  * mult_wise_c2d_re2d, namsa/probe_kernels.cu:180-222 and :126-134, launched at
@@ -132,6 +148,18 @@ int namsa_propagator(int Ny, int Nx, float kmax, float k_cut, float lambda, floa
  *   data: [B][Ny][Nx] complex64
  */
 int namsa_fft2d(void* data, int B, int Ny, int Nx, int inverse, void* stream);
+
+/*
+ * Thin-object approximation (replaces namsa/utils.py:130-135,
+ * thin_object_approx): out = |FFT2(probes * exp(i sum_s V_s))|, unshifted and
+ * unnormalised, the phase summed over slices in FP64.
+ *   V:      [S][Ny][Nx] float64
+ *   probes: [B][Ny][Nx] complex64
+ *   psi_ws: [B][Ny][Nx] complex64 workspace
+ *   out:    [B][Ny][Nx] float32
+ */
+int namsa_thin_object(const double* V, int S, const void* probes, int B, int Ny, int Nx, void* psi_ws, float* out,
+                      void* stream);
 
 /* sum |x|^2 over n complex64 values into *out (one float64 on the device);
  * the normalization of build_probe, namsa/msa.py:727. */

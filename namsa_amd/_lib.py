@@ -62,6 +62,8 @@ SIGNATURES = {
     "namsa_row_fft": (_i, [_vp, _i, _i, _i, _i, _f, _vp]),
     "namsa_precise_slices": (_i, [_i]),
     "namsa_sched_precise": (_i, [_i, _i, _i]),
+    "namsa_integrate_slices": (_i, [_vp, _vp, _i, _i, _i, _i, _d, _vp, _vp, _vp]),
+    "namsa_thin_object": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp]),
     "namsa_modes_combine": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "namsa_modes_expand": (_i, [_vp, _vp, _i, _i, _i, _vp]),
 }

@@ -146,6 +146,29 @@ int check_band(int Nx, int& lo, int& hi, const char* who) {
   return kOk;
 }
 
+// Thin-object exit wave probes * exp(i sum_s V_s) (namsa/utils.py:130-135),
+// the phase summed over slices in slice order in FP64.
+__global__ void __launch_bounds__(256) thin_object_kernel(const double* V, int S, const float2* probes, int B,
+                                                          long long plane, float2* psi) {
+  const long long p = blockIdx.x * 256LL + threadIdx.x;
+  if (p >= plane) return;
+  double ph = 0.0;
+  for (int s = 0; s < S; ++s) ph += V[s * plane + p];
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  for (int b = 0; b < B; ++b) {
+    const float2 q = probes[b * plane + p];
+    psi[b * plane + p] = make_float2(static_cast<float>(q.x * cs - q.y * sn), static_cast<float>(q.x * sn + q.y * cs));
+  }
+}
+
+__global__ void __launch_bounds__(256) abs_kernel(const float2* psi, long long n, float* out) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * 256) {
+    const float2 q = psi[i];
+    out[i] = sqrtf(q.x * q.x + q.y * q.y);
+  }
+}
+
 int check_grid(int Ny, int Nx, const char* who) {
   if (!supported(Ny) || !supported(Nx))
     return fail(kErrUnsupported, "%s: grid %dx%d is not a compiled transform size", who, Ny, Nx);
@@ -171,6 +194,24 @@ int namsa_fft2d(void* data, int B, int Ny, int Nx, int inverse, void* stream) {
   }
   if (int rc = col_pass(Ny, COL_INV, a, st)) return rc;
   return row_pass(Nx, ROW_INV, r, st);
+}
+
+int namsa_thin_object(const double* V, int S, const void* probes, int B, int Ny, int Nx, void* psi_ws, float* out,
+                      void* stream) {
+  if (int rc = check_grid(Ny, Nx, "namsa_thin_object")) return rc;
+  if (!V || !probes || !psi_ws || !out || S < 0 || B <= 0)
+    return fail(kErrArg, "namsa_thin_object: bad arguments (S=%d B=%d)", S, B);
+  hipStream_t st = as_stream(stream);
+  const long long plane = static_cast<long long>(Ny) * Nx;
+  hipLaunchKernelGGL(thin_object_kernel, dim3(static_cast<unsigned>((plane + 255) / 256)), dim3(256), 0, st, V, S,
+                     static_cast<const float2*>(probes), B, plane, static_cast<float2*>(psi_ws));
+  if (int rc = check_hip(hipGetLastError(), "thin_object_kernel launch")) return rc;
+  if (int rc = namsa_fft2d(psi_ws, B, Ny, Nx, 0, stream)) return rc;
+  const long long n = plane * B;
+  const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(abs_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                     static_cast<const float2*>(psi_ws), n, out);
+  return check_hip(hipGetLastError(), "abs_kernel launch");
 }
 
 int namsa_row_fft(void* data, int B, int Ny, int Nx, int inverse, float scale, void* stream) {

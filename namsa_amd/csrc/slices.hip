@@ -180,9 +180,61 @@ build_slices_kernel(const float* __restrict__ tables, int nZ, int Py, int Px,
   }
 }
 
+// ------------------------------------------------- sampled 3D potentials
+// Slice s of a potential sampled at z[0..Z) integrates samples [s d, (s+1) d)
+// with the trapezoid rule on their own coordinates and negates the result
+// (MSA.integrate_potential_slices, namsa/msa.py:390-407: np.split into S
+// chunks, trapz over each, V = -integral; no interval spans two chunks), and
+// t = exp(i sigma V) (MSAGPU.integrate_potential_slices, msa.py:568-571).
+// Each term is (z1 - z0) * (p0 + p1) / 2, summed in z order without
+// contraction, as numpy's trapz evaluates it.
+__global__ void __launch_bounds__(256) integrate_slices_kernel(const double* pot, const double* z, int d,
+                                                               long long plane, double sigma, float2* t_out,
+                                                               double* v_out) {
+#pragma clang fp contract(off)
+  const long long p = blockIdx.x * 256LL + threadIdx.x;
+  if (p >= plane) return;
+  const long long s = blockIdx.y;
+  const double* c = pot + s * d * plane + p;
+  const double* zs = z + s * d;
+  double acc = 0.0;
+  double prev = c[0];
+  for (int j = 1; j < d; ++j) {
+    const double cur = c[j * plane];
+    acc += (zs[j] - zs[j - 1]) * (prev + cur) / 2.0;
+    prev = cur;
+  }
+  const double v = -acc;
+  const long long o = s * plane + p;
+  if (v_out) v_out[o] = v;
+  if (t_out) {
+    double sn, cs;
+    sincos(sigma * v, &sn, &cs);
+    t_out[o] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+  }
+}
+
 }  // namespace namsa
 
 using namespace namsa;
+
+extern "C" {
+
+int namsa_integrate_slices(const double* pot, const double* z, int S, int d, int Ny, int Nx, double sigma,
+                           void* t_out, double* v_out, void* stream) {
+  if (!pot || !z || (!t_out && !v_out) || S < 0 || d < 1 || Ny <= 0 || Nx <= 0)
+    return fail(kErrArg, "namsa_integrate_slices: bad arguments (S=%d d=%d Ny=%d Nx=%d)", S, d, Ny, Nx);
+  if (S == 0) return kOk;
+  if (S > 65535) return fail(kErrArg, "namsa_integrate_slices: S=%d exceeds the grid y limit", S);
+  const long long plane = static_cast<long long>(Ny) * Nx;
+  const long long blocks = (plane + 255) / 256;
+  if (blocks > 0x7fffffffLL) return fail(kErrArg, "namsa_integrate_slices: plane too large");
+  hipLaunchKernelGGL(integrate_slices_kernel, dim3(static_cast<unsigned>(blocks), S), dim3(256), 0,
+                     as_stream(stream), pot, z, d, plane, sigma, static_cast<float2*>(t_out), v_out);
+  return check_hip(hipGetLastError(), "integrate_slices_kernel launch");
+}
+
+}  // extern "C"
 
 extern "C" {
 

@@ -164,6 +164,8 @@ class MSAGPU:
         Ny, Nx = self._grid
         if not (lib.namsa_fft_supported(Ny) and lib.namsa_fft_supported(Nx)):
             raise NamsaError(f"sampling {Ny}x{Nx} has no compiled transform schedule")
+        if self._t is not None and tuple(self._t.shape[1:]) != (Ny, Nx):
+            raise NamsaError(f"potential slices are {tuple(self._t.shape[1:])}, the sampling is {Ny}x{Nx}")
 
     # ------------------------------------------------------------------ device
     def setup_device(self, gpu_id=0):
@@ -245,6 +247,58 @@ class MSAGPU:
         self.pot_dev_ptr = self._t
         self._slices_host = None
         self.print_verbose("Built %d potential slices with shape:%s pixels" % (self.num_slices, (Ny, Nx)))
+
+    def integrate_potential_slices(self, pot, grid, slice_thickness, output=False):
+        """Transmission slices from a sampled 3D potential, e.g. an all-electron
+        ab initio one (msa.py:373-424 and :568-578).
+
+        pot: [z][y][x] samples; grid: (z, y, x) coordinate arrays in Angstrom.
+        As in the reference, the plane is cropped to the square min(Nx, Ny),
+        S = int(z extent // slice_thickness), d = Z // S samples per slice
+        (z cropped to S*d when d differs from ceil(Z/S)), and slice s is the
+        negated trapezoid integral over its own d samples (no interval spans
+        two slices); t = exp(i sigma V) is formed on the GPU
+        (namsa_integrate_slices, FP64 integration).  Updates num_slices, dims
+        (coordinate maxima of the full grids), sampling, kmax, max_ang,
+        pix_size and kpix_size as msa.py:409-421 does.  output=True returns
+        V [S][M][M] float64.
+        """
+        torch = _torch()
+        dev = self._dev()
+        z, y, x = (np.asarray(c, dtype=np.float64) for c in grid)
+        pot = np.asarray(pot)
+        self.slice_t = float(slice_thickness)
+        S = int((z[-1] - z[0]) // slice_thickness)
+        if S < 1:
+            raise ValueError(f"z extent {z[-1] - z[0]} holds no slice of {slice_thickness}")
+        if pot.shape[0] != z.size:
+            raise ValueError(f"potential has {pot.shape[0]} z samples, the grid {z.size}")
+        M = min(x.size, y.size)
+        d = int(pot.shape[0] // S)
+        pot_d = torch.from_numpy(np.ascontiguousarray(pot[: d * S, :M, :M], dtype=np.float64)).to(dev)
+        z_d = torch.from_numpy(np.ascontiguousarray(z[: d * S])).to(dev)
+        self._t = torch.empty((S, M, M), dtype=torch.complex64, device=dev)
+        v_d = torch.empty((S, M, M), dtype=torch.float64, device=dev) if output else None
+        call("namsa_integrate_slices", ptr(pot_d), ptr(z_d), S, d, M, M, float(self.sigma), ptr(self._t), ptr(v_d),
+             stream_handle())
+        del pot_d, z_d
+        self.pot_dev_ptr = self._t
+        self._slices_host = None
+        self.num_slices = np.int32(S)
+        self.dims = np.array([x.max(), y.max(), z.max()])
+        if self.max_angle is None:
+            self.sampling = np.array([M, M], dtype=np.int64)
+            self.kmax = float(np.min(self.sampling / self.dims[:2]))
+            self.max_ang = self.kmax * self.Lambda
+        else:
+            self.max_ang = self.max_angle
+            self.kmax = self.max_ang / self.Lambda
+            self.sampling = np.floor(self.kmax * self.dims[:2]).astype(np.int64)[::-1]
+        self.pix_size = self.dims[:2][::-1] / self.sampling
+        self.kpix_size = self.kmax / self.sampling
+        if output:
+            return v_d.cpu().numpy()
+        return None
 
     @property
     def potential_slices(self):

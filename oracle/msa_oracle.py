@@ -40,6 +40,12 @@ Pinning (tests/test_oracle.py):
                                       shifted CBED, brute-force DFT at N = 8
   probe_positions .................. notebook KAT 17x17 = 289, explicit grid
   detector_mask_reference .......... brute-force annulus from pixel radii
+  integrate_slices ................. constant and linear-in-z closed forms,
+                                      the intervals between chunks left out,
+                                      z crop rule on a 13-sample grid
+  integration_params ............... worked example of msa.py:409-421
+  thin_object ...................... vacuum = |FFT(probe)|, brute-force DFT
+                                      at N = 8
 """
 from __future__ import annotations
 
@@ -382,3 +388,60 @@ def integrate_cbed(cbeds, grid_steps, det):
     """sum(CBED * detector) per raster point, shaped grid_steps (msa.py:436-439)."""
     shape = list(np.asarray(grid_steps).astype(int)) + list(cbeds.shape[-2:])
     return np.sum(cbeds.reshape(shape) * det, axis=(-2, -1))
+
+
+
+# ---------------------------------------------------- alternate slice sources
+def integrate_slices(pot, grid, slice_thickness):
+    """Slices of a sampled 3D potential (MSA.integrate_potential_slices,
+    msa.py:373-407): crop the plane to the square min(Nx, Ny) (msa.py:386-390),
+    S = int(z extent // dz), d = Z // S, crop z to S*d samples when d differs
+    from ceil(Z/S) (msa.py:393-399), split into S chunks of d samples, trapz
+    over each chunk with its own coordinates (no interval spans two chunks),
+    negate (msa.py:402-405).
+
+    pot [Z][Ny][Nx] (spatial order z, y, x), grid = (z, y, x) coordinate arrays.
+    Returns (V [S][M][M] float64, dict(num_slices, d, z_used, max_dim)).
+    """
+    z, y, x = (np.asarray(c, dtype=np.float64) for c in grid)
+    S = int((z[-1] - z[0]) // slice_thickness)
+    M = min(x.size, y.size)
+    pot = np.asarray(pot, dtype=np.float64)[:, :M, :M]
+    d = int(pot.shape[0] // S)
+    if d != int(np.ceil(pot.shape[0] / S)):
+        pot = pot[: d * S]
+        z = z[: d * S]
+    V = np.empty((S, M, M))
+    for s in range(S):
+        c, zc = pot[s * d:(s + 1) * d], z[s * d:(s + 1) * d]
+        acc = np.zeros((M, M))
+        for j in range(1, d):   # numpy's trapz terms, summed in z order
+            acc = acc + (zc[j] - zc[j - 1]) * (c[j] + c[j - 1]) / 2.0
+        V[s] = -acc
+    return V, {"num_slices": S, "d": d, "z_used": d * S, "max_dim": M}
+
+
+def integration_params(slices_shape, grid, lam, max_angle=None):
+    """Simulation parameters after integrate_potential_slices (msa.py:409-421):
+    dims from the coordinate maxima of the full grids, sampling from the slice
+    shape (or from kmax * dims when max_angle is set), kmax, max_ang, pix_size
+    and kpix_size."""
+    z, y, x = (np.asarray(c, dtype=np.float64) for c in grid)
+    dims = np.array([x.max(), y.max(), z.max()])
+    if max_angle is None:
+        sampling = np.array(slices_shape[1:])
+        kmax = np.min(sampling / dims[:2])
+        max_ang = kmax * lam
+    else:
+        max_ang = max_angle
+        kmax = max_ang / lam
+        sampling = np.floor(kmax * dims[:2]).astype(np.int64)[::-1]
+    pix_size = dims[:2][::-1] / sampling
+    return {"dims": dims, "sampling": sampling, "kmax": kmax, "max_ang": max_ang, "pix_size": pix_size,
+            "kpix_size": kmax / sampling}
+
+
+def thin_object(V, probes):
+    """|FFT2(probes * exp(i sum_s V_s))| (thin_object_approx, utils.py:130-135)."""
+    psi = np.asarray(probes, dtype=np.complex128) * np.exp(1j * np.asarray(V, dtype=np.float64).sum(0))
+    return np.abs(np.fft.fft2(psi))

@@ -305,3 +305,91 @@ def test_cpu_port_matches_oracle_where_conventions_coincide():
     cb = O.cbed(psi_k, np.zeros((1, 2), np.float32), km, t, PM)[0]
     rel = np.linalg.norm(cb_cpu - cb) / np.linalg.norm(cb)
     assert rel < 2e-4
+
+
+# ------------------------------------------------------------------ alternate slice sources
+def _grid3(nz, ny, nx, dz=0.25, dy=0.1, dx=0.12):
+    return np.arange(nz) * dz, np.arange(ny) * dy, np.arange(nx) * dx
+
+
+def test_integrate_slices_closed_forms():
+    # constant c: each slice integrates c over its own (d - 1) intervals
+    z, y, x = _grid3(17, 6, 5)      # extent 4.0, dz = 1.0 -> S = 4, d = 4, 16 of 17 samples used
+    c = 2.5
+    V, info = O.integrate_slices(np.full((17, 6, 5), c), (z, y, x), 1.0)
+    assert info == {"num_slices": 4, "d": 4, "z_used": 16, "max_dim": 5}
+    assert V.shape == (4, 5, 5)
+    np.testing.assert_allclose(V, -c * 3 * 0.25)
+    # linear in z (trapezoid rule exact): p = a + b z -> -(a dz' + b (z1^2 - z0^2) / 2) per slice
+    a, b = 1.5, -0.75
+    p = (a + b * z)[:, None, None] * np.ones((1, 6, 5))
+    V, _ = O.integrate_slices(p, (z, y, x), 1.0)
+    for s in range(4):
+        z0, z1 = z[4 * s], z[4 * s + 3]
+        np.testing.assert_allclose(V[s], -(a * (z1 - z0) + b * (z1 ** 2 - z0 ** 2) / 2), rtol=1e-13)
+
+
+def test_integrate_slices_leave_out_intervals_between_chunks():
+    # Z = 12 samples, S = 3, d = 4: the chunk integrals miss the intervals
+    # 3-4 and 7-8 of the full trapezoid integral
+    rng = np.random.default_rng(1)
+    z = np.cumsum(rng.uniform(0.2, 0.4, 12))
+    z -= z[0]
+    y, x = np.arange(4) * 0.1, np.arange(4) * 0.1
+    p = rng.normal(size=(12, 4, 4))
+    dz = (z[-1] - z[0]) / 3 - 1e-9
+    V, info = O.integrate_slices(p, (z, y, x), dz)
+    assert info["num_slices"] == 3 and info["d"] == 4
+    full = integrate.trapezoid(p, x=z, axis=0)
+    gaps = sum((z[k + 1] - z[k]) * (p[k] + p[k + 1]) / 2 for k in (3, 7))
+    np.testing.assert_allclose(-V.sum(0), full - gaps, rtol=1e-12, atol=1e-12)
+    # and each slice is the library trapezoid over its own samples
+    for s in range(3):
+        np.testing.assert_allclose(-V[s], integrate.trapezoid(p[4 * s:4 * s + 4], x=z[4 * s:4 * s + 4], axis=0),
+                                   rtol=1e-13, atol=1e-14)
+
+
+def test_integrate_slices_crop_rules():
+    # non-square planes crop to the square min(Nx, Ny); Z = 13, extent 12, dz = 4:
+    # S = 3, d = 4 != ceil(13/3) = 5 -> the 13th sample is dropped
+    z, y, x = _grid3(13, 7, 9, dz=1.0)
+    p = np.random.default_rng(2).normal(size=(13, 7, 9))
+    V, info = O.integrate_slices(p, (z, y, x), 4.0)
+    assert info == {"num_slices": 3, "d": 4, "z_used": 12, "max_dim": 7}
+    np.testing.assert_allclose(-V[2], integrate.trapezoid(p[8:12, :7, :7], x=z[8:12], axis=0), rtol=1e-13,
+                               atol=1e-14)
+    # Z divisible by S: nothing dropped
+    z, y, x = _grid3(12, 4, 4, dz=1.0)
+    _, info = O.integrate_slices(np.zeros((12, 4, 4)), (z, y, x), 3.6)   # extent 11 -> S = 3, d = 4
+    assert info["z_used"] == 12
+
+
+def test_integration_params_worked_example():
+    # msa.py:409-421 by hand: dims from the full-grid maxima, sampling from the
+    # slice shape, kmax = min(sampling / dims[:2]) (sampling is (y, x), dims (x, y))
+    z, y, x = np.arange(9) * 0.5, np.arange(300) * 0.05, np.arange(256) * 0.04
+    pr = O.integration_params((4, 256, 256), (z, y, x), 0.0250795)
+    np.testing.assert_allclose(pr["dims"], [255 * 0.04, 299 * 0.05, 4.0])
+    assert list(pr["sampling"]) == [256, 256]
+    assert pr["kmax"] == pytest.approx(min(256 / 10.2, 256 / 14.95))
+    assert pr["max_ang"] == pytest.approx(pr["kmax"] * 0.0250795)
+    np.testing.assert_allclose(pr["pix_size"], [14.95 / 256, 10.2 / 256])
+    # max_angle given: sampling = floor(kmax * dims[:2]) reversed
+    pr = O.integration_params((4, 256, 256), (z, y, x), 0.0250795, max_angle=0.2)
+    k = 0.2 / 0.0250795
+    assert list(pr["sampling"]) == [int(np.floor(k * 14.95)), int(np.floor(k * 10.2))]
+
+
+def test_thin_object_vacuum_and_brute_force():
+    rng = np.random.default_rng(3)
+    N = 8
+    probes = rng.normal(size=(2, N, N)) + 1j * rng.normal(size=(2, N, N))
+    np.testing.assert_allclose(O.thin_object(np.zeros((3, N, N)), probes), np.abs(np.fft.fft2(probes)))
+    V = rng.normal(size=(3, N, N))
+    psi = probes[1] * np.exp(1j * (V[0] + V[1] + V[2]))
+    jj, kk = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    F = np.zeros((N, N), complex)
+    for u in range(N):
+        for v in range(N):
+            F[u, v] = np.sum(psi * np.exp(-2j * np.pi * (u * jj + v * kk) / N))
+    np.testing.assert_allclose(O.thin_object(V, probes)[1], np.abs(F), rtol=1e-12, atol=1e-12)
