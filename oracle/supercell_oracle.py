@@ -1,0 +1,217 @@
+"""Test infrastructure only: a CPU restatement of the reference's structure
+front-end (namsa/supercell.py), written as the reference's per-site loops.
+Used to check namsa_amd.supercell, which vectorises the same steps. Nothing in
+the product imports this module.
+
+The reference reads CIFs through pymatgen (CifParser, Lattice, SymmOp; not
+installed here, unpinned in setup.py:11). Two of pymatgen's published
+conventions are restated here: the lattice matrix of
+Lattice.from_parameters, and the affine matrix of an 'x, y, z' operation
+string (evaluated arithmetically here, not parsed by pattern).
+
+Pins (tests/test_supercell.py):
+- lattice_from_parameters: the reference's triclinic fixture
+  cif_files/mvc-9418_P_1_1_1.xyz, row norms equal to a, b, c, and
+  angles recovered.
+- symop: the 192 Fd-3m operations of cif_files/Si.cif give the 8
+  diamond positions (closed form); a composition check (W1·W2 of two
+  strings equals the string of the composed map).
+- transform + search_sites: fixtures mvc-9418_A_1_1_1.xyz,
+  mvc-9418_A_3_3_3.xyz, and mvc-9418_P_3_3_3.xyz (the search run at
+  P·3).
+- unit_cell: mvc-9418_P_1_1_1.xyz and diamond Si.
+- orthogonal_supercell: supercell_mvc-9418.xyz (726 atoms; reproduced by
+  the "lattice" projection rule, which that fixture was made with), the
+  closed-form frame of the default projections on the triclinic cell for
+  the reference's element-wise rule, and Si supercells (both rules agree
+  on a cubic basis) against a direct enumeration of the diamond lattice in
+  the rotated box.
+"""
+import ast
+import operator
+import re
+
+import numpy as np
+
+SITES_DTYPE = [("atom_type", "|U16"), ("atomic_number", "i"), ("frac_x", "f"), ("frac_y", "f"),
+               ("frac_z", "f"), ("occ", "f"), ("DW", "f")]
+SUPERCELL_DTYPE = [("atom_type", "|U16"), ("atomic_number", "i"), ("x", "f"), ("y", "f"), ("z", "f"),
+                   ("occ", "f"), ("DW", "f")]
+
+CENTERING = {"P": np.identity(3),
+             "I": np.array([[-1, 1, 1], [1, -1, 1], [1, 1, -1]]),
+             "A": np.array([[1, 0, 0], [0, 1, -1], [0, 1, 1]]),
+             "C": np.array([[1, 1, 0], [-1, 1, 0], [0, 0, 1]]),
+             "R": np.array([[2, -1, -1], [1, 1, -2], [1, 1, 1]]),
+             "F": np.array([[0, 1, 1], [1, 0, 1], [1, 1, 0]])}
+
+
+def lattice_from_parameters(a, b, c, alpha, beta, gamma):
+    """Lattice vectors as rows, pymatgen Lattice.from_parameters (vesta=False)."""
+    al, be, ga = (np.radians(v) for v in (alpha, beta, gamma))
+    v = (np.cos(al) * np.cos(be) - np.cos(ga)) / (np.sin(al) * np.sin(be))
+    gs = np.arccos(max(-1.0, min(1.0, v)))
+    return np.array([[a * np.sin(be), 0.0, a * np.cos(be)],
+                     [-b * np.sin(al) * np.cos(gs), b * np.sin(al) * np.sin(gs), b * np.cos(al)],
+                     [0.0, 0.0, c]])
+
+
+_OPS = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, ast.Div: operator.truediv}
+
+
+def _eval(node, env):
+    if isinstance(node, ast.Expression):
+        return _eval(node.body, env)
+    if isinstance(node, ast.Constant):
+        return float(node.value)
+    if isinstance(node, ast.Name):
+        return env[node.id]
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        v = _eval(node.operand, env)
+        return -v if isinstance(node.op, ast.USub) else v
+    if isinstance(node, ast.BinOp) and type(node.op) in _OPS:
+        return _OPS[type(node.op)](_eval(node.left, env), _eval(node.right, env))
+    raise ValueError("unsupported symmetry-operation syntax")
+
+
+def symop(xyz):
+    """4x4 affine matrix of 'x+1/2, -y, z', by evaluating each component at
+    the origin and the unit vectors (the map is affine)."""
+    text = re.sub(r"(\d)([xyz])", r"\1*\2", xyz.lower())          # '2x' -> '2*x'
+    comps = [ast.parse(c.strip(), mode="eval") for c in text.split(",")]
+    assert len(comps) == 3
+    pts = {"o": (0.0, 0.0, 0.0), "x": (1.0, 0.0, 0.0), "y": (0.0, 1.0, 0.0), "z": (0.0, 0.0, 1.0)}
+    val = {k: [_eval(c, dict(zip("xyz", p))) for c in comps] for k, p in pts.items()}
+    m = np.identity(4)
+    for r in range(3):
+        m[r, 3] = val["o"][r]
+        for k, ax in enumerate("xyz"):
+            m[r, k] = val[ax][r] - val["o"][r]
+    return m
+
+
+def make_sites(symbols, Z, frac, occ, dw=0.07):
+    return np.array([(s, z, f[0], f[1], f[2], o, dw) for s, z, f, o in zip(symbols, Z, frac, occ)],
+                    dtype=SITES_DTYPE)
+
+
+def search_sites(sites, Q, n):
+    """namsa/supercell.py:211-230, one site at a time."""
+    n_x, n_y, n_z = n
+    out = []
+    for site in sites:
+        xs = np.linspace(site["frac_x"], site["frac_x"] + n_x, n_x + 1)
+        ys = np.linspace(site["frac_y"], site["frac_y"] + n_y, n_y + 1)
+        zs = np.linspace(site["frac_z"], site["frac_z"] + n_z, n_z + 1)
+        X, Y, Z = np.meshgrid(xs, ys, zs)
+        pts = np.column_stack((X.flatten(), Y.flatten(), Z.flatten(), np.ones(X.size)))
+        f = np.dot(Q, pts.T)[:3, :].T
+        f -= f // 1
+        _, idx = np.unique(np.round(f, 2), axis=0, return_index=True)
+        f = np.round(f[idx], 6)
+        f = f[(f < 1.0).all(axis=1)]
+        out.append(np.array([(site["atom_type"], site["atomic_number"], a, b, c, site["occ"], site["DW"])
+                             for a, b, c in f], dtype=SITES_DTYPE))
+    return np.concatenate(out)
+
+
+def transform(basis, sym_ops, sites, centering="P", scaling=(1, 1, 1), rotation=None, origin_shift=(0, 0, 0),
+              max_iter=4, primitive=False):
+    """namsa/supercell.py:158-244. Returns (basis, lattice_const, angles_deg,
+    sym_ops, sites)."""
+    scaling = np.array(scaling)
+    if rotation is None:
+        rotation = CENTERING[centering] * (1.0 if not primitive else (1 / 3 if centering == "R" else 0.5)) \
+            if centering != "P" else CENTERING["P"]
+    p = np.array(rotation)
+    for k in range(3):
+        p[:, k] = p[:, k] * scaling[k]
+    P = np.identity(4)
+    P[:3, :3] = p
+    P[:3, 3] = origin_shift
+    Q = np.linalg.inv(P)
+    basis = np.dot(basis, p)
+    lc = np.linalg.norm(basis, axis=0)
+    ang = np.rad2deg([np.arccos(np.dot(basis.T[i], basis.T[j]) / (lc[i] * lc[j])) for i, j in [[1, 2], [2, 0], [0, 1]]])
+    sym_ops = [Q @ w @ P for w in sym_ops]
+    if centering != "P":
+        old, it, n = 0, 0, scaling.astype(int).copy()
+        while True:
+            new = search_sites(sites, Q, n)
+            if new.size == old or it >= max_iter:
+                sites = new
+                break
+            old, it = new.size, it + 1
+            n = n + 1
+    return basis, lc, ang, sym_ops, sites
+
+
+def unit_cell(basis, sym_ops, sites):
+    """namsa/supercell.py:246-283. Returns (fractional records, Cartesian
+    positions, atom types)."""
+    recs, carts, types = [], [], []
+    for site in sites:
+        sp = np.array([site["frac_x"], site["frac_y"], site["frac_z"], 1])
+        pos = np.array([np.dot(w, sp)[:3] for w in sym_ops])
+        pos -= pos // 1
+        _, idx = np.unique(np.round(pos, 2), return_index=True, axis=0)
+        pos = pos[idx]
+        for f in pos:
+            recs.append((site["atom_type"], site["atomic_number"], f[0], f[1], f[2], site["occ"], site["DW"]))
+            types.append(site["atom_type"])
+        carts.append(np.dot(basis, pos.T).T)
+    return np.array(recs, dtype=SITES_DTYPE), np.concatenate(carts), np.array(types)
+
+
+def cartesian_basis(lc, ang_deg):
+    """namsa/supercell.py:321-335."""
+    a, b, c = lc
+    al, be, ga = np.asarray(ang_deg) * np.pi / 180
+    cx, cy = c * np.cos(be), c * np.cos(al) * np.sin(ga)
+    return np.array([[a, b * np.cos(ga), cx], [0, b * np.sin(ga), cy], [0, 0, np.sqrt(abs(c * c - cx * cx - cy * cy))]])
+
+
+def orthogonal_supercell(lc, ang_deg, cell, size=(20.0, 20.0, 20.0), projec_1=(1, 0, 0), projec_2=(0, 0, 1),
+                         rule="components"):
+    """namsa/supercell.py:368-455 with the per-site loop; returns the
+    structured records sorted as np.sort sorts them. rule "components" is the
+    reference's element-wise projection (:377-380); "lattice" takes the axes
+    along sum_i p_i a_i, the rule the reference's supercell fixture was
+    made with."""
+    B = cartesian_basis(lc, ang_deg)
+    Bn = B / np.linalg.norm(B, axis=0)
+    if rule == "components":
+        b = np.array([np.array(projec_1) * v for v in B.T]).sum(0)
+        c = np.array([np.array(projec_2) * v for v in B.T]).sum(0)
+    else:
+        b = sum(p * v for p, v in zip(projec_1, B.T))
+        c = sum(p * v for p, v in zip(projec_2, B.T))
+    b = b / np.linalg.norm(b)
+    c = c / np.linalg.norm(c)
+    a = np.cross(b, c)
+    if np.round(np.dot(b, c), 5) != 0.0:
+        b = np.cross(a, c)
+    T = np.dot(Bn.T, np.column_stack([a, b, c])).T
+    size = np.array(size, dtype=float)
+    sa, sb, sc = size
+    corners = np.unique([[sa, 0, 0], [0, sb, 0], [0, 0, sc], [sa, sb, 0], [sa, 0, sc], [0, sb, sc], [sa, sb, sc],
+                         [0, 0, 0]], axis=0)
+    org = np.dot(np.linalg.inv(T), corners.T).T
+    hi = np.ceil(org.max(0) / lc).astype(int)
+    lo = np.floor(org.min(0) / lc).astype(int)
+    rep = hi - lo
+    ta, tb, tc = np.dot(T, np.diag(lc)).T
+    tv = np.column_stack([ta, tb, tc])
+    bn = np.linalg.norm(B, axis=0)
+    out = []
+    for site in cell:
+        p0 = np.dot(T, np.array([site["frac_x"], site["frac_y"], site["frac_z"]]) * bn)
+        gx = np.linspace(lo[0], hi[0], num=rep[0] + 1)
+        gy = np.linspace(lo[1], hi[1], num=rep[1] + 1)
+        gz = np.linspace(lo[2], hi[2], num=rep[2] + 1)
+        X, Y, Z = np.meshgrid(gx, gy, gz)
+        pos = np.dot(np.column_stack([X.flatten(), Y.flatten(), Z.flatten()]), tv.T) + p0
+        pos = pos[np.logical_and(pos >= 0, pos <= size).all(axis=1)]
+        out.append(np.array([(site["atom_type"], site["atomic_number"], x, y, z, site["occ"], site["DW"])
+                             for x, y, z in pos], dtype=SUPERCELL_DTYPE))
+    return np.sort(np.concatenate(out), axis=0)

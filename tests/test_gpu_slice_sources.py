@@ -108,3 +108,36 @@ def test_thin_object_matches_oracle(gpu):
     assert err < 3e-6        # FP32 rounding of an N^2-point transform
     one = thin_object_approx(V, probes[1])
     assert one.shape == (N, N) and rel_l2(one, ref[1]) < 3e-6
+
+
+def test_cif_supercell_multislice(gpu):
+    """Structure front-end feeding the GPU path: Si.cif (192 operations) ->
+    unit cell -> a 22 x 22 x 16 Å [110]/[001] supercell -> MSAGPU slices and
+    CBEDs, against the oracle's slices and CBEDs for the same records."""
+    import os
+
+    from namsa_amd.supercell import SupercellBuilder
+    from tests.parity_util import oracle_pm, oracle_probe, oracle_slices
+
+    _say("Si.cif -> SupercellBuilder -> MSAGPU, 256^2")
+    sp = SupercellBuilder(os.path.join(os.path.dirname(__file__), "golden", "cif_files", "Si.cif"))
+    sp.build_unit_cell()
+    sp.make_orthogonal_supercell(supercell_size=np.array([22.0, 22.0, 16.0]), projec_1=np.array([1, 1, 0]),
+                                 projec_2=np.array([0, 0, 1]))
+    sim = MSAGPU(200, 0.025, sp.supercell_sites, sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    pos = np.array([[0.0, 0.0], [1.92, -0.96]])
+    sim.build_probe(dict(DEFAULT_PROBE))
+    sim.probe_positions = pos.copy()
+    sim.num_probes = np.int32(len(pos))
+    sim.grid_steps = np.array([len(pos), 1])
+    sim.plan_simulation()
+    sim.multislice(bandwidth=1 / 3)
+    t_ref, _ = oracle_slices(sim, 2.0)
+    psi_k, _ = oracle_probe(sim)
+    cb_ref = O.cbed(psi_k, pos.astype(np.float32), np.float32(sim.kmax), t_ref, oracle_pm(sim, 2.0, 1 / 3))
+    errs = [rel_l2(sim.probes[i], cb_ref[i]) for i in range(len(pos))]
+    _say(f"  {len(sp.supercell_sites)} atoms, S={sim.num_slices}, CBED relL2 {['%.2e' % e for e in errs]}")
+    assert max(errs) < 1e-4
