@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 1 GiB device copy first (HBM counter calibration)")
     p.add_argument("--kernel-reps", type=int, default=40)
-    p.add_argument("--workload", choices=["config2", "config3", "config4", "config5", "potential3d"],
+    p.add_argument("--workload", choices=["config2", "config3", "config4", "config5", "potential3d", "records"],
                    default="config2",
                    help="config2: 4D-STEM at 512^2 (headline); config3: 48-material uneven batch; "
                         "config4: learned coarse-slice corrector vs plain at 256^2; config5: 2048^2 "
@@ -251,6 +251,74 @@ def bench_potential3d(args):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_records(args):
+    """Training records of a 4D-STEM dataset (process_cbed): 4096 CBEDs of
+    512^2 float32 already in HBM, standardised per pattern to half. One call
+    is one step; the CPU baseline is the oracle's numpy float32 restatement of
+    the reference's process_cbed on a bounded sample."""
+    import torch
+
+    import namsa_amd._lib as L
+
+    P, N = 4096, 512
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    cb = (torch.rand((P, N, N), generator=g, device=dev) ** 6).contiguous()     # heavy-tailed, nonnegative
+    out = torch.empty((P, N, N), dtype=torch.float16, device=dev)
+    lib = L.load()
+    ws = torch.empty(int(lib.namsa_records_workspace_doubles(P, N, N)), dtype=torch.float64, device=dev)
+    st = L.stream_handle()
+
+    def step():
+        L.call("namsa_cbed_records", L.ptr(cb), P, N, N, L.ptr(out), L.ptr(ws), st)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(1, args.steps)
+    e0.record()
+    for _ in range(reps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 6 * P * N * N                      # one float32 read and one half write per pixel
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    value = P / (ms * 1e-3)
+    log(f"records: {P} x {N}^2 CBEDs -> half in {ms:.3f} ms, {gbs:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import records_oracle as R
+
+        host = cb[:64].cpu().numpy()
+        passes, t0 = 0, time.perf_counter()
+        while True:                              # about 10 s of CPU work
+            R.process_cbed(host)
+            passes += 1
+            dt = time.perf_counter() - t0
+            if dt >= 10.0:
+                break
+        cpu = {"value": passes * 64 / dt, "unit": "CBEDs/s", "cores": 1, "kind": "port",
+               "sample": f"{passes} passes over 64 CBEDs of {N}^2 (oracle/records_oracle.py process_cbed, numpy "
+                         f"{np.__version__}, one process), {dt:.1f} s"}
+        log(f"cpu baseline {cpu['value']:.4g} CBEDs/s")
+    out_line = {
+        "metric": "CBEDs encoded into standardised half records per second", "value": value,
+        "unit": "CBEDs/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (heavy-tailed uniform^6 patterns)",
+        "config": {"workload": f"process_cbed: {P} CBEDs of {N}x{N} float32 -> float16", "probes": P,
+                   "grid": [N, N]},
+        "roofline": {"bound": "hbm", "kernel": "rec_stats_kernel + rec_standardize_kernel", "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": nbytes, "avg_launch_ms": ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out_line), flush=True)
 
 
 def bench_config3(args):
@@ -504,6 +572,8 @@ def main():
         return bench_config5(args)
     if args.workload == "potential3d":
         return bench_potential3d(args)
+    if args.workload == "records":
+        return bench_records(args)
 
     copy_gbs = None
     if args.copy_calibration:

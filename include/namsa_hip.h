@@ -303,6 +303,36 @@ int namsa_modes_expand(const float* Ys, float* E, int C, int B, int m, void* str
 int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B, int hw, float s,
                           void* stream);
 
+/* ---- training records (the 4D-STEM dataset step after the slice loop) ---- */
+
+/*
+ * Per-CBED standardisation to IEEE half (replaces process_cbed,
+ * scripts/summit_scripts/utils.py:104-118): out = (x - mean) / std per
+ * [Ny][Nx] item, population std, float32 arithmetic for the subtraction and
+ * division, round-to-nearest-even to half. A constant item gives NaN, as
+ * numpy's 0/0 does.
+ *   cbed: [P][Ny][Nx] float32;  out_f16: [P][Ny][Nx] half (uint16 bits)
+ *   ws:   namsa_records_workspace_doubles(P, Ny, Nx) doubles
+ */
+long long namsa_records_workspace_doubles(int P, int Ny, int Nx);
+int namsa_cbed_records(const float* cbed, int P, int Ny, int Nx, void* out_f16, double* ws, void* stream);
+
+/*
+ * Projected-potential training record (replaces process_potential,
+ * scripts/summit_scripts/utils.py:74-102): mean over slices of imag(t),
+ * scipy.ndimage.gaussian_filter(sigma, truncate, mode 'reflect'), the crop
+ * rows and columns [Ny/4, min(3Nx/4, dim)), standardised with the deviation
+ * floored at 1/sqrt(size), to half.
+ *   t:       [S][Ny][Nx] complex64 transmission slices
+ *   out_f16: [rows][cols] half, rows/cols from namsa_potential_record_shape
+ *   ws:      namsa_potential_record_workspace_bytes(Ny, Nx) bytes
+ * The reference uses sigma = 1.2 and scipy's default truncate = 4.0.
+ */
+int namsa_potential_record_shape(int Ny, int Nx, int* rows, int* cols);
+long long namsa_potential_record_workspace_bytes(int Ny, int Nx);
+int namsa_potential_record(const void* t, int S, int Ny, int Nx, double sigma, double truncate, void* out_f16,
+                           void* ws, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,11 @@ SIGNATURES = {
     "namsa_sched_precise": (_i, [_i, _i, _i]),
     "namsa_integrate_slices": (_i, [_vp, _vp, _i, _i, _i, _i, _d, _vp, _vp, _vp]),
     "namsa_thin_object": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp]),
+    "namsa_records_workspace_doubles": (_ll, [_i, _i, _i]),
+    "namsa_cbed_records": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
+    "namsa_potential_record_shape": (_i, [_i, _i, _vp, _vp]),
+    "namsa_potential_record_workspace_bytes": (_ll, [_i, _i]),
+    "namsa_potential_record": (_i, [_vp, _i, _i, _i, _d, _d, _vp, _vp, _vp]),
     "namsa_modes_combine": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "namsa_modes_expand": (_i, [_vp, _vp, _i, _i, _i, _vp]),
 }

@@ -1,0 +1,120 @@
+"""Training-record kernels on MI355X against the oracle: per-CBED
+standardisation to half (namsa_cbed_records) and the projected-potential
+record (namsa_potential_record).
+
+The oracle follows the reference's numpy float32 path, whose mean and
+deviation carry float32 summation errors (up to about 1e-6 relative). The
+kernels compute them in FP64. Near y = 0 a half ulp is far smaller than
+that error, so the bar has three parts:
+- every value within one half ulp of the oracle's, plus 1e-5 (1 + |y|);
+- at least 99% bit-identical, with NaNs in the same places;
+- for CBEDs, within half a half-ulp of the exactly standardised value
+  (FP64 statistics), plus the float32 rounding of (x - mean) / std.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from namsa_amd import records as REC
+from oracle import records_oracle as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _half_ulp(v):
+    """Spacing of IEEE half at |v| (2^-24 in the subnormal range)."""
+    a = np.abs(np.asarray(v, dtype=np.float64))
+    e = np.floor(np.log2(np.maximum(a, 2.0 ** -14)))
+    return 2.0 ** (e - 10)
+
+
+def _check(got, ref, label, exact=None, m_over_s=None):
+    assert got.shape == ref.shape and got.dtype == np.float16
+    nan = np.isnan(ref)
+    assert (np.isnan(got) == nan).all(), label
+    g, r = got[~nan].astype(np.float64), ref[~nan].astype(np.float64)
+    same = float((got[~nan].view(np.uint16) == ref[~nan].view(np.uint16)).mean()) if g.size else 1.0
+    excess = np.abs(g - r) - (_half_ulp(r) + 1e-5 * (1 + np.abs(r)))
+    _say(f"  {label}: max |got-ref| {np.abs(g - r).max() if g.size else 0:.3g}, {100 * same:.3f}% identical")
+    assert g.size == 0 or excess.max() <= 0
+    assert same >= 0.99
+    if exact is not None:
+        ex = exact[~nan]
+        tol = 0.5 * _half_ulp(ex) * (1 + 1e-3) + 1e-6 * (1 + np.abs(ex) + m_over_s[~nan])
+        worst = (np.abs(g - ex) - tol).max()
+        _say(f"  {label}: vs exact, worst margin {worst:.3g}")
+        assert worst <= 0
+
+
+def _exact_cbed(x):
+    x64 = x.astype(np.float64)
+    m = x64.mean(axis=(1, 2), keepdims=True)
+    s = x64.std(axis=(1, 2), keepdims=True)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return (x64 - m) / s, np.broadcast_to(np.abs(m) / s, x.shape)
+
+
+@pytest.mark.parametrize("P,Ny,Nx", [(5, 256, 256), (3, 96, 130), (2, 7, 9)])
+def test_cbed_records_match_oracle(gpu, P, Ny, Nx):
+    _say(f"namsa_cbed_records P={P} {Ny}x{Nx}")
+    rng = np.random.default_rng(P + Ny)
+    x = (rng.exponential(size=(P, Ny, Nx)) ** 3).astype(np.float32)
+    x[0, Ny // 2, Nx // 2] = 1e4                                  # a bright central disk pixel
+    if P > 2:
+        x[2] = 0.125                                              # a constant item: NaN
+    ex, ms = _exact_cbed(x)
+    _check(REC.process_cbed(x), R.process_cbed(x), "cbed", ex, ms)
+
+
+def test_cbed_records_many_batches_on_device(gpu):
+    """More items than one MALL-sized batch (96 MB of float32), encoded from a
+    device tensor; the result stays on the device."""
+    import torch
+
+    _say("namsa_cbed_records P=200 256x256, device tensor")
+    rng = np.random.default_rng(9)
+    x = rng.gamma(0.5, size=(200, 256, 256)).astype(np.float32)
+    xd = torch.from_numpy(x).to(gpu)
+    out = REC.process_cbed(xd, as_tensor=True)
+    assert out.is_cuda and out.dtype == torch.float16
+    ex, ms = _exact_cbed(x)
+    _check(out.cpu().numpy(), R.process_cbed(x), "cbed batches", ex, ms)
+
+
+@pytest.mark.parametrize("S,Ny,Nx", [(4, 256, 256), (3, 200, 256), (2, 64, 48)])
+def test_potential_record_matches_oracle(gpu, S, Ny, Nx):
+    _say(f"namsa_potential_record S={S} {Ny}x{Nx}")
+    rng = np.random.default_rng(S * Ny)
+    yy, xx = np.mgrid[0:Ny, 0:Nx]
+    V = np.stack([np.sin(0.07 * (s + 1) * xx) * np.cos(0.05 * yy) + 0.3 * rng.normal(size=(Ny, Nx))
+                  for s in range(S)])
+    t = np.exp(1j * 0.4 * V).astype(np.complex64)
+    _check(REC.process_potential(t), R.process_potential(t), "potential")
+
+
+def test_records_of_a_simulation(gpu):
+    """The dataset step as the reference's script runs it: multislice at
+    256^2, then both records from the simulation's device buffers."""
+    from namsa_amd import DEFAULT_PROBE, MSAGPU
+    from tests.parity_util import random_block
+
+    _say("records of a 256^2 simulation, 16 positions")
+    sim = MSAGPU(100, 0.02, random_block(3, 24.0, 12.0, 400), sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    sim.build_probe(dict(DEFAULT_PROBE))
+    sim.generate_probe_positions(grid_steps=np.array([4, 4]))
+    sim.plan_simulation()
+    sim.multislice(bandwidth=2.0 / 3)
+    cb = REC.process_cbed(sim.probes_device)
+    pot = REC.process_potential(sim.potential_slices_device)
+    assert cb.shape == (16, 256, 256) and pot.shape == (1, 128, 128)
+    _check(cb, R.process_cbed(sim.probes), "simulation cbed")
+    _check(pot, R.process_potential(sim.potential_slices), "simulation potential")
