@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--probes", type=int, default=64, help="raster edge (probes = edge^2)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-probes", type=int, default=0, help="CPU sample size (0: 2 per core)")
+    p.add_argument("--cpu-probes", type=int, default=0, help="CPU sample size (0: 5 per core)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 1 GiB device copy first (HBM counter calibration)")
     p.add_argument("--kernel-reps", type=int, default=40)
@@ -159,7 +159,7 @@ def cpu_baseline(sim, n_probes):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    n = n_probes or 2 * cores
+    n = n_probes or 5 * cores                  # five points per core: about 10 s of CPU work
     idx = np.linspace(0, len(sim.probe_positions) - 1, n).astype(int)
     # the CPU path takes positions as (x, y) (msa.py:225)
     pos = sim.probe_positions[idx][:, ::-1]

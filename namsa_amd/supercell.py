@@ -14,6 +14,7 @@ lexsort, so a supercell of a million atoms takes a fraction of a second.
 Host-side code. The records are consumed by `MSAGPU(..., supercell=...)`,
 whose slice builder runs on the GPU.
 """
+import os
 import re
 from warnings import warn
 
@@ -86,9 +87,9 @@ def read_cif(path_or_text):
     """The first data block of a CIF as {tag: value or list of values} (all
     strings, tags lower-cased), as the first entry of pymatgen's
     CifParser.as_dict() (namsa/supercell.py:95-96)."""
-    text = path_or_text
-    if "\n" not in path_or_text:
-        with open(path_or_text) as f:
+    text = os.fspath(path_or_text)
+    if "\n" not in text:
+        with open(text) as f:
             text = f.read()
     data, seen_block = {}, False
     toks = list(_tokens(text))
