@@ -90,11 +90,6 @@ def symop(xyz):
     return m
 
 
-def make_sites(symbols, Z, frac, occ, dw=0.07):
-    return np.array([(s, z, f[0], f[1], f[2], o, dw) for s, z, f, o in zip(symbols, Z, frac, occ)],
-                    dtype=SITES_DTYPE)
-
-
 def search_sites(sites, Q, n):
     """namsa/supercell.py:211-230, one site at a time."""
     n_x, n_y, n_z = n
