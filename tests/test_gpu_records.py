@@ -72,9 +72,8 @@ def test_cbed_records_match_oracle(gpu, P, Ny, Nx):
     _check(REC.process_cbed(x), R.process_cbed(x), "cbed", ex, ms)
 
 
-def test_cbed_records_many_batches_on_device(gpu):
-    """More items than one MALL-sized batch (96 MB of float32), encoded from a
-    device tensor; the result stays on the device."""
+def test_cbed_records_on_device(gpu):
+    """200 patterns encoded from a device tensor; the result stays on the device."""
     import torch
 
     _say("namsa_cbed_records P=200 256x256, device tensor")
@@ -84,7 +83,16 @@ def test_cbed_records_many_batches_on_device(gpu):
     out = REC.process_cbed(xd, as_tensor=True)
     assert out.is_cuda and out.dtype == torch.float16
     ex, ms = _exact_cbed(x)
-    _check(out.cpu().numpy(), R.process_cbed(x), "cbed batches", ex, ms)
+    _check(out.cpu().numpy(), R.process_cbed(x), "cbed device", ex, ms)
+
+
+def test_cbed_records_beyond_one_launch(gpu):
+    """70000 patterns of 4 x 6: more items than one launch's grid holds (65535)."""
+    _say("namsa_cbed_records P=70000 4x6")
+    rng = np.random.default_rng(12)
+    x = rng.exponential(size=(70000, 4, 6)).astype(np.float32)
+    ex, ms = _exact_cbed(x)
+    _check(REC.process_cbed(x), R.process_cbed(x), "cbed 2 launches", ex, ms)
 
 
 @pytest.mark.parametrize("S,Ny,Nx", [(4, 256, 256), (3, 200, 256), (2, 64, 48)])
