@@ -246,11 +246,21 @@ def bench_potential3d(args):
                                f"of {d} samples, exp(i sigma V) to complex64", "grid": [N, N], "samples": Z,
                    "slices": S},
         "roofline": {"bound": "hbm", "kernel": "integrate_slices_kernel", "achieved": gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": nbytes,
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": _aux_traffic("potential3d"),
+                     "bytes_per_launch": nbytes,
                      "avg_launch_ms": ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def _aux_traffic(workload):
+    """HBM bytes per call measured with the FETCH_SIZE / WRITE_SIZE counters
+    (profiles/aux_workloads_hbm_traffic.txt), or None."""
+    tf = ROOT / "profiles" / "hbm_traffic_aux.json"
+    if not tf.exists():
+        return None
+    return json.loads(tf.read_text()).get(workload, {}).get("traffic")
 
 
 def bench_records(args):
@@ -314,8 +324,8 @@ def bench_records(args):
         "config": {"workload": f"process_cbed: {P} CBEDs of {N}x{N} float32 -> float16", "probes": P,
                    "grid": [N, N]},
         "roofline": {"bound": "hbm", "kernel": "rec_stats_kernel + rec_standardize_kernel", "achieved": gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": nbytes, "avg_launch_ms": ms},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": _aux_traffic("records"), "bytes_per_launch": nbytes, "avg_launch_ms": ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out_line), flush=True)
