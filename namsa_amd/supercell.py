@@ -501,10 +501,14 @@ class SupercellBuilder:
         orthorhombic cells).
 
         As in the reference, each unit-cell site is placed at trans_mat ·
-        (frac ⊙ |cell axes|). It is translated by trans_mat · diag(a, b, c)
-        times every integer triple in the repeat range, and kept when
+        (frac ⊙ |cell axes|). trans_mat holds the normalised cell axes, so
+        this is the site's Cartesian position rotated into the box frame. It is
+        translated by trans_mat · diag(a, b, c) (the lattice vectors in that
+        frame) times every integer triple in the repeat range, and kept when
         0 ≤ x, y, z ≤ size (bounds inclusive, in float64, before the float32
-        records).
+        records). The repeat range spans the fractional coordinates of the box
+        corners, floored and ceiled, which covers every lattice point of the
+        box for any cell shape.
         """
         trans_mat, t_vec, lo, hi, bnorm, size = self._supercell_frame(supercell_size, projec_1, projec_2,
                                                                       projection_rule)

@@ -308,3 +308,88 @@ def test_xyz_files_round_trip(tmp_path):
     assert len(arr) == len(back)
     assert sp.to_xyz(str(tmp_path / "uc.xyz"))                      # unit cell, Cartesian
     assert (tmp_path / "uc.xyz").read_text().splitlines()[0] == "8"
+
+
+GAN_CIF = """data_GaN
+_cell_length_a 3.189
+_cell_length_b 3.189
+_cell_length_c 5.185
+_cell_angle_alpha 90
+_cell_angle_beta 90
+_cell_angle_gamma 120
+_symmetry_space_group_name_H-M 'P 63 m c'
+loop_
+_symmetry_equiv_pos_as_xyz
+'x, y, z'
+'-y, x-y, z'
+'-x+y, -x, z'
+'-x, -y, z+1/2'
+'y, -x+y, z+1/2'
+'x-y, x, z+1/2'
+'-y, -x, z'
+'-x+y, y, z'
+'x, x-y, z'
+'y, x, z+1/2'
+'x-y, -y, z+1/2'
+'-x, -x+y, z+1/2'
+loop_
+_atom_site_label
+_atom_site_type_symbol
+_atom_site_fract_x
+_atom_site_fract_y
+_atom_site_fract_z
+_atom_site_occupancy
+Ga1 Ga 0.333333 0.666667 0.0 1
+N1 N 0.333333 0.666667 0.377 1
+"""
+
+
+def _wurtzite_frac(u=0.377):
+    return np.array([[1 / 3, 2 / 3, 0.0], [2 / 3, 1 / 3, 0.5], [1 / 3, 2 / 3, u], [2 / 3, 1 / 3, 0.5 + u]])
+
+
+def test_hexagonal_unit_cell_is_wurtzite(tmp_path):
+    """P6_3mc operations on the two GaN sites give the four wurtzite
+    positions (Ga at 2b, N at 2b + u)."""
+    p = tmp_path / "gan.cif"
+    p.write_text(GAN_CIF)
+    sp = S.SupercellBuilder(str(p))
+    sp.build_unit_cell()
+    uc = sp.unit_cell_positions
+    frac = np.column_stack([uc["frac_x"], uc["frac_y"], uc["frac_z"]]).astype(np.float64)
+    want = _wurtzite_frac()
+    assert list(uc["atom_type"]) == ["Ga", "Ga", "N", "N"]
+    got = sorted(map(tuple, np.round(frac, 5)))
+    assert got == sorted(map(tuple, np.round(want, 5)))
+    np.testing.assert_allclose(sp.lattice_angles, [90, 90, 120], atol=1e-12)
+
+
+def test_hexagonal_supercell_against_lattice_enumeration(tmp_path):
+    """A non-orthogonal cell: the supercell equals the wurtzite crystal
+    (lattice points + basis, Cartesian), rotated into the box frame and
+    enumerated directly, for the lattice-vector projection rule. Also
+    checks that the box frame is a rotation and the nearest Ga-N distance."""
+    p = tmp_path / "gan.cif"
+    p.write_text(GAN_CIF)
+    sp = S.SupercellBuilder(str(p))
+    sp.build_unit_cell()
+    size = np.array([12.0, 10.0, 11.0])
+    sp.make_orthogonal_supercell(supercell_size=size, projec_1=np.array([1, 0, 0]), projec_2=np.array([0, 0, 1]),
+                                 projection_rule="lattice")
+    got = _pos(sp.supercell_sites)
+    T, _, _, _, _, _ = sp._supercell_frame(size, [1, 0, 0], [0, 0, 1], "lattice")
+    cart = sp.CartesianBasis()
+    R = T @ np.linalg.inv(cart / np.linalg.norm(cart, axis=0))      # box frame from Cartesian
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+    n = np.arange(-8, 9)
+    cells = np.stack(np.meshgrid(n, n, n, indexing="ij"), -1).reshape(-1, 3)
+    pts = ((cells[:, None, :] + _wurtzite_frac()[None]) @ cart.T).reshape(-1, 3) @ R.T
+    eps = 1e-6
+    sure = pts[((pts > eps) & (pts < size - eps)).all(1)]
+    maybe = pts[((pts > -eps) & (pts < size + eps)).all(1)]
+    from scipy.spatial import cKDTree
+    assert (cKDTree(got).query(sure)[0] < 1e-4).all()
+    assert (cKDTree(maybe).query(got)[0] < 1e-4).all()
+    ga, nn = got[sp.supercell_sites["atom_type"] == "Ga"], got[sp.supercell_sites["atom_type"] == "N"]
+    a, c, u = 3.189, 5.185, 0.377                  # wurtzite bonds: u c along c, sqrt(a^2/3 + (1/2 - u)^2 c^2) basal
+    assert abs(cKDTree(nn).query(ga)[0].min() - min(u * c, np.sqrt(a * a / 3 + ((0.5 - u) * c) ** 2))) < 1e-4
