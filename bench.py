@@ -300,6 +300,26 @@ def bench_records(args):
     gbs = nbytes / (ms * 1e-3) / 1e9
     value = P / (ms * 1e-3)
     log(f"records: {P} x {N}^2 CBEDs -> half in {ms:.3f} ms, {gbs:.0f} GB/s")
+    # The same records when the CBED readout kept each pattern's sums (multislice(moments=True)):
+    # one pass. The sums are formed here, before the timed region, in 256-pattern slabs.
+    mom = torch.empty((P, 2), dtype=torch.float64, device=dev)
+    for p0 in range(0, P, 256):
+        slab = cb[p0:p0 + 256].double()
+        mom[p0:p0 + 256, 0] = slab.sum(dim=(1, 2))
+        mom[p0:p0 + 256, 1] = (slab * slab).sum(dim=(1, 2))
+        del slab
+    out1 = torch.empty_like(out)
+    L.call("namsa_cbed_records_from_moments", L.ptr(cb), L.ptr(mom), P, N, N, L.ptr(out1), st)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        L.call("namsa_cbed_records_from_moments", L.ptr(cb), L.ptr(mom), P, N, N, L.ptr(out1), st)
+    e1.record()
+    torch.cuda.synchronize()
+    ms1 = e0.elapsed_time(e1) / reps
+    same = float((out1.view(torch.int16) == out.view(torch.int16)).double().mean())
+    log(f"records from readout moments: {ms1:.3f} ms, {nbytes / (ms1 * 1e-3) / 1e9:.0f} GB/s, "
+        f"{100 * same:.3f}% identical to the two-pass records")
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import records_oracle as R
@@ -325,7 +345,10 @@ def bench_records(args):
                    "grid": [N, N]},
         "roofline": {"bound": "hbm", "kernel": "rec_stats_kernel + rec_standardize_kernel", "achieved": gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": _aux_traffic("records"), "bytes_per_launch": nbytes, "avg_launch_ms": ms},
+                     "traffic": _aux_traffic("records"), "bytes_per_launch": nbytes, "avg_launch_ms": ms,
+                     "from_readout_moments": {"kernel": "rec_standardize_kernel", "avg_launch_ms": ms1,
+                                              "achieved": nbytes / (ms1 * 1e-3) / 1e9,
+                                              "frac": nbytes / (ms1 * 1e-3) / 1e9 / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out_line), flush=True)

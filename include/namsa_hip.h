@@ -241,6 +241,20 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
                      int nDet, double* det_ws, double* det_out, void* stream);
 
 /*
+ * namsa_multislice in CBED mode that also returns, per probe, the sums of
+ * the CBED and of its square (FP64, taken in the readout from the values it
+ * writes). They are the statistics process_cbed needs
+ * (scripts/summit_scripts/utils.py:104-118), so that
+ * namsa_cbed_records_from_moments encodes the records in one pass.
+ *   mom_ws:  namsa_det_workspace_doubles(group or B, Ny, Nx, 2) doubles
+ *   mom_out: [B][2] doubles (sum, sum of squares)
+ */
+int namsa_multislice_moments(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                             const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                             float out_scale, void* psi_ws, float* cbed, const float* det, int nDet,
+                             double* det_ws, double* det_out, double* mom_ws, double* mom_out, void* stream);
+
+/*
  * Row transforms of planes in place: forward (unscaled, scale must be 1) or
  * inverse times scale.  The slice state between steps is the row transform
  * of the wave, so the inverse with scale 1/Nx gives the wave in real space
@@ -316,6 +330,9 @@ int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B,
  */
 long long namsa_records_workspace_doubles(int P, int Ny, int Nx);
 int namsa_cbed_records(const float* cbed, int P, int Ny, int Nx, void* out_f16, double* ws, void* stream);
+/* The same records from the moments namsa_multislice_moments returned: one pass. */
+int namsa_cbed_records_from_moments(const float* cbed, const double* moments, int P, int Ny, int Nx,
+                                    void* out_f16, void* stream);
 
 /*
  * Projected-potential training record (replaces process_potential,

@@ -42,6 +42,8 @@ SIGNATURES = {
     "namsa_integrate": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp]),
     "namsa_multislice": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _i, _i, _i, _f, _vp, _vp,
                               _vp, _i, _vp, _vp, _vp]),
+    "namsa_multislice_moments": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _i,
+                                      _vp, _vp, _vp, _vp, _vp]),
     "namsa_sched_count": (_i, [_i, _i]),
     "namsa_sched_name": (C.c_char_p, [_i, _i, _i]),
     "namsa_sched_get": (_i, [_i, _i]),
@@ -66,6 +68,7 @@ SIGNATURES = {
     "namsa_thin_object": (_i, [_vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp]),
     "namsa_records_workspace_doubles": (_ll, [_i, _i, _i]),
     "namsa_cbed_records": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
+    "namsa_cbed_records_from_moments": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "namsa_potential_record_shape": (_i, [_i, _i, _vp, _vp]),
     "namsa_potential_record_workspace_bytes": (_ll, [_i, _i]),
     "namsa_potential_record": (_i, [_vp, _i, _i, _i, _d, _d, _vp, _vp, _vp]),

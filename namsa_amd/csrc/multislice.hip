@@ -299,10 +299,14 @@ int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int ba
   return col_pass(Ny, COL_PROP, a, as_stream(stream));
 }
 
-int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
-                     const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
-                     int mode, float out_scale, void* psi_ws, float* cbed, const float* det,
-                     int nDet, double* det_ws, double* det_out, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax, const void* t,
+                    int S, const void* pm, int band_lo, int band_hi, int group, int mode, float out_scale,
+                    void* psi_ws, float* cbed, const float* det, int nDet, double* det_ws, double* det_out,
+                    double* mom_ws, double* mom_out, void* stream) {
   if (int rc = check_grid(Ny, Nx, "namsa_multislice")) return rc;
   if (B <= 0 || S < 0 || group < 0 || !psi_k || !pos_yx || !psi_ws)
     return fail(kErrArg, "namsa_multislice: B=%d S=%d group=%d or null probe/positions/workspace",
@@ -314,6 +318,8 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
   if (mode == NAMSA_OUT_CBED && !cbed) return fail(kErrArg, "namsa_multislice: null cbed");
   if (mode == NAMSA_OUT_CBED && nDet > 0 && (!det || !det_ws || !det_out))
     return fail(kErrArg, "namsa_multislice: null detector buffers");
+  if (mom_ws && (mode != NAMSA_OUT_CBED || !mom_out))
+    return fail(kErrArg, "namsa_multislice_moments: moments need CBED output and a moments buffer");
   if (int rc = check_band(Nx, band_lo, band_hi, "namsa_multislice")) return rc;
 
   hipStream_t st = as_stream(stream);
@@ -378,6 +384,7 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
     a.det = det;
     a.n_det = nDet > 0 ? nDet : 0;
     a.det_ws = det_ws;
+    a.mom_ws = mom_ws;
     if (int rc = col_pass(Ny, COL_READOUT, a, st, precise)) return rc;
     if (nDet > 0) {
       const int n = Bg * nDet;
@@ -385,8 +392,35 @@ int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int 
                          a.strips_per_image, nDet, Bg, det_out + static_cast<long long>(g0) * nDet);
       if (int rc = check_hip(hipGetLastError(), "det_finalize_kernel launch")) return rc;
     }
+    if (mom_ws) {  // the same strip-order sum, two "detectors": the CBED and its square
+      const int n = Bg * 2;
+      hipLaunchKernelGGL(det_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, st, mom_ws,
+                         a.strips_per_image, 2, Bg, mom_out + 2LL * g0);
+      if (int rc = check_hip(hipGetLastError(), "moments finalize launch")) return rc;
+    }
   }
   return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int namsa_multislice(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                     const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                     int mode, float out_scale, void* psi_ws, float* cbed, const float* det,
+                     int nDet, double* det_ws, double* det_out, void* stream) {
+  return multislice_impl(psi_k, pos_yx, B, Ny, Nx, kmax, t, S, pm, band_lo, band_hi, group, mode, out_scale,
+                         psi_ws, cbed, det, nDet, det_ws, det_out, nullptr, nullptr, stream);
+}
+
+int namsa_multislice_moments(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                             const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                             float out_scale, void* psi_ws, float* cbed, const float* det, int nDet,
+                             double* det_ws, double* det_out, double* mom_ws, double* mom_out, void* stream) {
+  if (!mom_ws || !mom_out) return fail(kErrArg, "namsa_multislice_moments: null moments buffers");
+  return multislice_impl(psi_k, pos_yx, B, Ny, Nx, kmax, t, S, pm, band_lo, band_hi, group, NAMSA_OUT_CBED,
+                         out_scale, psi_ws, cbed, det, nDet, det_ws, det_out, mom_ws, mom_out, stream);
 }
 
 }  // extern "C"

@@ -42,6 +42,7 @@ struct ColArgs {
   const float* det;      // [n_det][Ny][Nx] (READOUT)
   int n_det;
   double* det_ws;        // [B][strips_per_image][n_det] (READOUT)
+  double* mom_ws;        // [B][strips_per_image][2]: sums of the CBED and of its square (READOUT), or null
 };
 
 // MI355X has 8 XCDs, each with its own L2; the dispatcher deals workgroups

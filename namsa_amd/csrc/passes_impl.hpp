@@ -284,6 +284,26 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
         inten[m] = (y.x * y.x + y.y * y.y) * a.amp;
         a.cbed[b * plane + off] = inten[m];
       }
+      if (a.mom_ws) {  // per-strip moments for the record encoder (FP64; the squares are exact)
+        double s[2] = {0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          s[0] += double(inten[m]);
+          s[1] += double(inten[m]) * double(inten[m]);
+        }
+        for (int k = 0; k < 2; ++k) {
+          const double w = wave_sum(s[k]);
+          if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+          __syncthreads();
+          if (threadIdx.x == 0) {
+            double tot = 0.0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) tot += red[i];
+            a.mom_ws[(b * a.strips_per_image + strip) * 2 + k] = tot;
+          }
+          __syncthreads();
+        }
+      }
       for (int d = 0; d < a.n_det; ++d) {
         const float* w = a.det + d * plane;
         double s = 0.0;

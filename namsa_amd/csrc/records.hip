@@ -110,11 +110,12 @@ __global__ __launch_bounds__(kRecThreads) void rec_stats_kernel(const float* __r
 
 __device__ __forceinline__ unsigned short to_half_bits(float y) { return __half_as_ushort(__float2half_rn(y)); }
 
-// grid (chunks, items): reduce the item's partials, then standardise the chunk to half.
+// grid (chunks, items): reduce the item's nparts partial sums, then standardise the chunk to half.
+// shifted: the sums are of (x - x0) with x0 the item's first value (rec_stats_kernel); otherwise of x.
 template <bool VEC>
 __global__ __launch_bounds__(kRecThreads) void rec_standardize_kernel(const float* __restrict__ x, long long n,
-                                                                      int chunks, const double* __restrict__ part,
-                                                                      double floor_std,
+                                                                      int nparts, const double* __restrict__ part,
+                                                                      int shifted, double floor_std,
                                                                       unsigned short* __restrict__ out) {
   __shared__ float stat[2];
   const int c = blockIdx.x;
@@ -122,16 +123,16 @@ __global__ __launch_bounds__(kRecThreads) void rec_standardize_kernel(const floa
   const float* xi = x + item * n;
   unsigned short* oi = out + item * n;
   double s = 0.0, ss = 0.0;
-  for (int i = threadIdx.x; i < chunks; i += kRecThreads) {
-    s += part[(item * chunks + i) * 2];
-    ss += part[(item * chunks + i) * 2 + 1];
+  for (int i = threadIdx.x; i < nparts; i += kRecThreads) {
+    s += part[(item * nparts + i) * 2];
+    ss += part[(item * nparts + i) * 2 + 1];
   }
   block_sum2(s, ss);
   if (threadIdx.x == 0) {
     const double dn = double(n);
     const double m = s / dn;
     const double var = ss / dn - m * m;
-    const float mean = float(double(xi[0]) + m);
+    const float mean = float((shifted ? double(xi[0]) : 0.0) + m);
     float sd = float(sqrt(var > 0.0 ? var : 0.0));
     if (double(sd) < floor_std) sd = float(floor_std);
     stat[0] = mean;
@@ -235,12 +236,12 @@ int standardize_items(const float* x, long long items, long long n, double floor
     unsigned short* ob = out + b0 * n;
     if (vec) {
       hipLaunchKernelGGL(rec_stats_kernel<true>, grid, dim3(kRecThreads), 0, st, xb, n, int(chunks), part);
-      hipLaunchKernelGGL(rec_standardize_kernel<true>, grid, dim3(kRecThreads), 0, st, xb, n, int(chunks), part,
+      hipLaunchKernelGGL(rec_standardize_kernel<true>, grid, dim3(kRecThreads), 0, st, xb, n, int(chunks), part, 1,
                          floor_std, ob);
     } else {
       hipLaunchKernelGGL(rec_stats_kernel<false>, grid, dim3(kRecThreads), 0, st, xb, n, int(chunks), part);
       hipLaunchKernelGGL(rec_standardize_kernel<false>, grid, dim3(kRecThreads), 0, st, xb, n, int(chunks), part,
-                         floor_std, ob);
+                         1, floor_std, ob);
     }
     const int rc = check_hip(hipGetLastError(), who);
     if (rc != kOk) return rc;
@@ -277,6 +278,32 @@ int namsa_cbed_records(const float* cbed, int P, int Ny, int Nx, void* out_f16, 
   if ((n + kRecChunk - 1) / kRecChunk > 0x7fffffffLL) return fail(kErrArg, "namsa_cbed_records: plane too large");
   return standardize_items(cbed, P, n, 0.0, static_cast<unsigned short*>(out_f16), ws, as_stream(stream),
                            "namsa_cbed_records launch");
+}
+
+int namsa_cbed_records_from_moments(const float* cbed, const double* moments, int P, int Ny, int Nx,
+                                    void* out_f16, void* stream) {
+  if (!cbed || !moments || !out_f16 || P < 0 || Ny <= 0 || Nx <= 0)
+    return fail(kErrArg, "namsa_cbed_records_from_moments: bad arguments (P=%d Ny=%d Nx=%d)", P, Ny, Nx);
+  const long long n = static_cast<long long>(Ny) * Nx;
+  const long long chunks = (n + kRecChunk - 1) / kRecChunk;
+  if (chunks > 0x7fffffffLL) return fail(kErrArg, "namsa_cbed_records_from_moments: plane too large");
+  unsigned short* out = static_cast<unsigned short*>(out_f16);
+  const bool vec = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(cbed) % 16 == 0) &&
+                   (reinterpret_cast<uintptr_t>(out) % 8 == 0);
+  hipStream_t st = as_stream(stream);
+  const long long batch = record_batch(n);
+  for (long long b0 = 0; b0 < P; b0 += batch) {
+    const long long nb = (P - b0 < batch) ? P - b0 : batch;
+    const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(nb));
+    if (vec)
+      hipLaunchKernelGGL(rec_standardize_kernel<true>, grid, dim3(kRecThreads), 0, st, cbed + b0 * n, n, 1,
+                         moments + 2 * b0, 0, 0.0, out + b0 * n);
+    else
+      hipLaunchKernelGGL(rec_standardize_kernel<false>, grid, dim3(kRecThreads), 0, st, cbed + b0 * n, n, 1,
+                         moments + 2 * b0, 0, 0.0, out + b0 * n);
+    if (int rc = check_hip(hipGetLastError(), "namsa_cbed_records_from_moments launch")) return rc;
+  }
+  return kOk;
 }
 
 int namsa_potential_record_shape(int Ny, int Nx, int* rows, int* cols) {

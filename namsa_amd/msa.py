@@ -414,7 +414,7 @@ class MSAGPU:
         return self.pm_device
 
     def multislice(self, bandwidth=1 / 3, unified_mem=False, batch_size=None, transmit=True,
-                   detectors=None, output="auto", group=None):
+                   detectors=None, output="auto", group=None, moments=False):
         """Propagate every probe position through all slices (msa.py:767-924).
 
         transmit=True: ``probes`` = float32 CBED [P][Ny][Nx], normalised by the
@@ -424,7 +424,10 @@ class MSAGPU:
         float weights) fuses the detector integrals into the readout; the
         result lands in ``self.detector_signals`` [P][nDet] (float64).
         ``group`` probes go through all slices together (default: as many as
-        fit the last-level cache, see ``cache_group``).
+        fit the last-level cache, see ``cache_group``).  moments=True (CBED
+        output) also keeps each CBED's sum and sum of squares, taken in the
+        readout, in ``cbed_moments_device`` [P][2] float64, for
+        ``records.process_cbed(..., moments=...)``.
         """
         torch = _torch()
         dev = self._dev()
@@ -470,6 +473,12 @@ class MSAGPU:
             det_ws = torch.empty(int(lib.namsa_det_workspace_doubles(chunk, Ny, Nx, nDet)),
                                  dtype=torch.float64, device=dev)
             self._det_dev = torch.empty((P, nDet), dtype=torch.float64, device=dev)
+        self._moments_dev = None
+        with_moments = bool(moments) and mode == _lib.OUT_CBED
+        if with_moments:
+            mom_ws = torch.empty(int(_lib.load().namsa_det_workspace_doubles(chunk, Ny, Nx, 2)),
+                                 dtype=torch.float64, device=dev)
+            self._moments_dev = torch.empty((P, 2), dtype=torch.float64, device=dev)
         st = stream_handle()
         t_arg = ptr(self._t) if S > 0 else None
         for start in range(0, P, chunk):
@@ -479,9 +488,14 @@ class MSAGPU:
             else:  # the wave state itself is the output
                 cbed_ptr, ws_ptr = None, ptr(out[start])
             det_args = (ptr(det), nDet, ptr(det_ws), ptr(self._det_dev[start])) if nDet else (None, 0, None, None)
-            call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
-                 np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
-                 group, mode, scale, ws_ptr, cbed_ptr, *det_args, st)
+            if with_moments:
+                call("namsa_multislice_moments", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
+                     np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
+                     group, scale, ws_ptr, cbed_ptr, *det_args, ptr(mom_ws), ptr(self._moments_dev[start]), st)
+            else:
+                call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
+                     np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
+                     group, mode, scale, ws_ptr, cbed_ptr, *det_args, st)
         self._probes_dev = out
         self._probes_host = None
         self.detector_signals = self._det_dev.cpu().numpy() if nDet else None
@@ -500,6 +514,11 @@ class MSAGPU:
     @property
     def probes_device(self):
         return self._probes_dev
+
+    @property
+    def cbed_moments_device(self):
+        """[P][2] float64 (sum, sum of squares) of each CBED from multislice(moments=True), else None."""
+        return getattr(self, "_moments_dev", None)
 
     # ------------------------------------------------------------------ detectors
     def detector_mask(self, inner_angle, outer_angle, mode="reference"):

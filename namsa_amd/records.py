@@ -36,14 +36,17 @@ def _device_f32(x, device):
     return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(device)
 
 
-def process_cbed(cbed, normalize=True, scale=(-1, 1), fp16=False, new_shape=None, device=None, as_tensor=False):
+def process_cbed(cbed, normalize=True, scale=(-1, 1), fp16=False, new_shape=None, device=None, as_tensor=False,
+                 moments=None):
     """Standardise each CBED of [P][Ny][Nx] by its own mean and population
     deviation and return float16 (utils.py:104-118). The reference's keyword
     arguments are accepted and, as there, unused.
 
     cbed: numpy array or torch tensor (a CUDA tensor is encoded in place on
-    its device). Returns numpy float16, or the device tensor with
-    as_tensor=True.
+    its device). moments: the [P][2] (sum, sum of squares) of the same
+    CBEDs, as MSAGPU.multislice(moments=True) keeps them; then the encoder
+    reads the CBEDs once instead of twice. Returns numpy float16, or the
+    device tensor with as_tensor=True.
     """
     torch = _torch()
     _lib.load()
@@ -56,6 +59,14 @@ def process_cbed(cbed, normalize=True, scale=(-1, 1), fp16=False, new_shape=None
         raise NamsaError(f"process_cbed expects [P][Ny][Nx], got shape {tuple(x.shape)}")
     P, Ny, Nx = (int(v) for v in x.shape)
     out = torch.empty((P, Ny, Nx), dtype=torch.float16, device=dev)
+    if moments is not None:
+        mom = (moments if isinstance(moments, torch.Tensor) else torch.from_numpy(np.asarray(moments)))
+        mom = mom.to(device=dev, dtype=torch.float64).contiguous()
+        if tuple(mom.shape) != (P, 2):
+            raise NamsaError(f"moments must be [{P}][2], got {tuple(mom.shape)}")
+        with torch.cuda.device(dev):
+            call("namsa_cbed_records_from_moments", ptr(x), ptr(mom), P, Ny, Nx, ptr(out), stream_handle())
+        return out if as_tensor else out.cpu().numpy()
     ws = torch.empty(max(1, int(_lib.load().namsa_records_workspace_doubles(P, Ny, Nx))), dtype=torch.float64,
                      device=dev)
     with torch.cuda.device(dev):
