@@ -14,6 +14,10 @@
 // slower on MI355X (12-96 MB: 4.12-2.11 ms against 1.91 ms for 4096 items of
 // 512^2), because the launch count grows faster than the reads shrink.
 //
+// namsa_cbed_records_from_moments: the same records from each item's sum and
+// sum of squares (unshifted), as the CBED readout of namsa_multislice_moments
+// keeps them; only rec_standardize_kernel runs, so the CBEDs are read once.
+//
 // namsa_potential_record: process_potential (utils.py:74-102): the mean over
 // slices of imag(t), a Gaussian filter (sigma 1.2, truncate 4, 'reflect'
 // edges, axis 0 then axis 1, float32 between the passes, as
