@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 1 GiB device copy first (HBM counter calibration)")
     p.add_argument("--kernel-reps", type=int, default=40)
-    p.add_argument("--workload", choices=["config2", "config3", "config4", "config5", "potential3d", "records"],
+    p.add_argument("--workload", choices=["config2", "config3", "config4", "config5", "potential3d", "records", "stream"],
                    default="config2",
                    help="config2: 4D-STEM at 512^2 (headline); config3: 48-material uneven batch; "
                         "config4: learned coarse-slice corrector vs plain at 256^2; config5: 2048^2 "
@@ -250,6 +250,71 @@ def bench_potential3d(args):
                      "bytes_per_launch": nbytes,
                      "avg_launch_ms": ms},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_stream(args):
+    """The 512^2 workload (4096 positions, 150 slices) with its CBEDs delivered
+    to host memory: compute only; compute then a blocking copy to pinned
+    memory; and multislice(sink=...), which copies each finished chunk on a
+    second stream while the next chunk computes. The same for the float16
+    records (readout moments, one-pass encoder)."""
+    import torch
+
+    from namsa_amd import records as REC
+
+    sim, _ = setup(64)
+    sim.plan_simulation()
+    P, S = int(sim.num_probes), int(sim.num_slices)
+    delivered = [0]
+
+    def sink(start, arr):
+        delivered[0] += len(arr)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    pinned = torch.empty((P, 512, 512), dtype=torch.float32, pin_memory=True)
+    pinned16 = torch.empty((P, 512, 512), dtype=torch.float16, pin_memory=True)
+
+    def blocking():
+        sim.multislice(bandwidth=1 / 3)
+        pinned.copy_(sim.probes_device)
+
+    def blocking_records():
+        sim.multislice(bandwidth=1 / 3, moments=True)
+        pinned16.copy_(REC.process_cbed(sim.probes_device, moments=sim.cbed_moments_device, as_tensor=True))
+
+    for _ in range(max(1, args.warmup)):
+        sim.multislice(bandwidth=1 / 3, sink=sink)
+    reps = max(1, min(args.steps, 3))
+    t = {"compute_only": [], "blocking_copy": [], "streamed": [], "records_blocking": [], "records_streamed": []}
+    for _ in range(reps):
+        t["compute_only"].append(timed(lambda: sim.multislice(bandwidth=1 / 3)))
+        t["blocking_copy"].append(timed(blocking))
+        delivered[0] = 0
+        t["streamed"].append(timed(lambda: sim.multislice(bandwidth=1 / 3, sink=sink)))
+        assert delivered[0] == P
+        t["records_blocking"].append(timed(blocking_records))
+        delivered[0] = 0
+        t["records_streamed"].append(timed(lambda: sim.multislice(bandwidth=1 / 3, sink=sink, records=True)))
+        assert delivered[0] == P
+    ms = {k: float(np.median(v)) for k, v in t.items()}
+    log("stream: " + ", ".join(f"{k} {v:.1f} ms" for k, v in ms.items()))
+    out = {
+        "metric": "probe-positions·slices/s at 512² with every CBED delivered to host memory",
+        "value": P * S / (ms["streamed"] * 1e-3), "unit": "probe-positions·slices/s", "n_gpus": 1, "steps": reps,
+        "warmup": args.warmup, "ms_per_step": ms["streamed"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded amorphous block, 73,728 atoms)",
+        "config": {"workload": "config-2 multislice (4096 positions, 150 slices, 512^2) with CBEDs streamed to "
+                               "pinned host memory in chunks", "grid": [512, 512], "probes": P, "slices": S},
+        "timings_ms": ms,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
@@ -607,6 +672,8 @@ def main():
         return bench_potential3d(args)
     if args.workload == "records":
         return bench_records(args)
+    if args.workload == "stream":
+        return bench_stream(args)
 
     copy_gbs = None
     if args.copy_calibration:

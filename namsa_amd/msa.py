@@ -75,6 +75,60 @@ def cache_group(Ny, Nx):
     return max(1, int(LLC_BYTES * share // (8 * Ny * Nx)))
 
 
+class _ChunkStreamer:
+    """Copies finished chunks of the CBED output to pinned host memory on a
+    second stream (default priority) and hands them to a sink, one chunk
+    behind the compute stream, so copies and the sink overlap the next
+    chunk's slices. With moments, each chunk is first encoded into float16
+    records on the compute stream (namsa_cbed_records_from_moments)."""
+
+    def __init__(self, out, moments, chunk, sink):
+        torch = _torch()
+        self.out, self.moments, self.sink = out, moments, sink
+        P, Ny, Nx = (int(v) for v in out.shape)
+        self.Ny, self.Nx = Ny, Nx
+        dtype = torch.float16 if moments is not None else torch.float32
+        self.host = [torch.empty((chunk, Ny, Nx), dtype=dtype, pin_memory=True) for _ in range(2)]
+        self.enc = ([torch.empty((chunk, Ny, Nx), dtype=torch.float16, device=out.device) for _ in range(2)]
+                    if moments is not None else None)
+        self.copy_stream = torch.cuda.Stream(device=out.device)
+        self.done = [None, None]
+        self.pending = None
+        self.k = 0
+
+    def push(self, start, B):
+        torch = _torch()
+        slot = self.k % 2
+        self.k += 1
+        if self.done[slot] is not None:          # the slot's previous copy and sink have finished
+            self.done[slot].synchronize()
+        src = self.out[start:start + B]
+        if self.enc is not None:
+            call("namsa_cbed_records_from_moments", ptr(src), ptr(self.moments[start]), B, self.Ny, self.Nx,
+                 ptr(self.enc[slot]), stream_handle())
+            src = self.enc[slot][:B]
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.out.device))
+        self.copy_stream.wait_event(ready)
+        with torch.cuda.stream(self.copy_stream):
+            self.host[slot][:B].copy_(src, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(self.copy_stream)
+        self.done[slot] = ev
+        if self.pending is not None:
+            self._deliver(*self.pending)
+        self.pending = (slot, start, B)
+
+    def _deliver(self, slot, start, B):
+        self.done[slot].synchronize()
+        self.sink(start, self.host[slot][:B].numpy())
+
+    def finish(self):
+        if self.pending is not None:
+            self._deliver(*self.pending)
+            self.pending = None
+
+
 class MSAGPU:
     """Multislice on one MI355X behind the reference MSAGPU interface."""
 
@@ -414,7 +468,7 @@ class MSAGPU:
         return self.pm_device
 
     def multislice(self, bandwidth=1 / 3, unified_mem=False, batch_size=None, transmit=True,
-                   detectors=None, output="auto", group=None, moments=False):
+                   detectors=None, output="auto", group=None, moments=False, sink=None, records=False):
         """Propagate every probe position through all slices (msa.py:767-924).
 
         transmit=True: ``probes`` = float32 CBED [P][Ny][Nx], normalised by the
@@ -428,6 +482,15 @@ class MSAGPU:
         output) also keeps each CBED's sum and sum of squares, taken in the
         readout, in ``cbed_moments_device`` [P][2] float64, for
         ``records.process_cbed(..., moments=...)``.
+
+        sink(start, array): stream the CBEDs out while the simulation runs.
+        The probes go in chunks (``batch_size``, by default a multiple of
+        the group giving about eight chunks). Each chunk's CBEDs, or with
+        records=True its float16 records (encoded on the GPU from the
+        readout's moments), are copied to pinned host memory on a second
+        stream, while the next chunk computes. sink then gets the chunk's
+        first probe index and a host array, which is only valid during the
+        call. The device CBEDs stay available as ``probes_device``.
         """
         torch = _torch()
         dev = self._dev()
@@ -452,9 +515,13 @@ class MSAGPU:
             mode = _lib.OUT_EXIT
             S = int(self.num_slices)
         chunk = min(int(self.max_probes), P)
+        group = int(group) if group else cache_group(Ny, Nx)
+        if sink is not None and mode != _lib.OUT_CBED:
+            raise NamsaError("sink streams CBEDs: it needs transmit=True and CBED output")
+        if sink is not None and not batch_size:
+            chunk = min(chunk, group * max(1, -(-P // group) // 8))
         if batch_size:
             chunk = min(chunk, int(batch_size))
-        group = int(group) if group else cache_group(Ny, Nx)
         ws_planes = min(chunk, group)
         pos_all = torch.from_numpy(np.ascontiguousarray(self.probe_positions, dtype=np.float32)).to(dev)
         if mode == _lib.OUT_CBED:
@@ -474,13 +541,15 @@ class MSAGPU:
                                  dtype=torch.float64, device=dev)
             self._det_dev = torch.empty((P, nDet), dtype=torch.float64, device=dev)
         self._moments_dev = None
-        with_moments = bool(moments) and mode == _lib.OUT_CBED
+        with_moments = (bool(moments) or (sink is not None and records)) and mode == _lib.OUT_CBED
         if with_moments:
             mom_ws = torch.empty(int(_lib.load().namsa_det_workspace_doubles(chunk, Ny, Nx, 2)),
                                  dtype=torch.float64, device=dev)
             self._moments_dev = torch.empty((P, 2), dtype=torch.float64, device=dev)
         st = stream_handle()
         t_arg = ptr(self._t) if S > 0 else None
+        if sink is not None:
+            streamer = _ChunkStreamer(out, self._moments_dev if records else None, chunk, sink)
         for start in range(0, P, chunk):
             B = min(chunk, P - start)
             if mode == _lib.OUT_CBED:
@@ -496,6 +565,10 @@ class MSAGPU:
                 call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
                      np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
                      group, mode, scale, ws_ptr, cbed_ptr, *det_args, st)
+            if sink is not None:
+                streamer.push(start, B)
+        if sink is not None:
+            streamer.finish()
         self._probes_dev = out
         self._probes_host = None
         self.detector_signals = self._det_dev.cpu().numpy() if nDet else None
