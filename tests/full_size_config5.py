@@ -3,9 +3,9 @@
 Too long for the test suite (the FP64 oracle needs ~2000 FFTs of 2048^2), so
 it runs in two steps:
 
-  on the GPU:   python tests/full_size_config5.py gpu  gpurun_out/c5_full.npz
-  on any CPU:   python tests/full_size_config5.py oracle gpurun_out/c5_full.npz [oracle.npz]
-  later runs:   python tests/full_size_config5.py compare gpurun_out/c5_full.npz oracle.npz
+  on the GPU:   python tests/full_size_config5.py gpu  c5_full.npz
+  on any CPU:   python tests/full_size_config5.py oracle c5_full.npz [oracle.npz]
+  later runs:   python tests/full_size_config5.py compare c5_full.npz oracle.npz
 
 The first writes the GPU CBEDs of two probe positions (MSAGPU, dz = 1 A,
 bandwidth 1/3); the second rebuilds the same structure, streams the oracle
