@@ -393,3 +393,21 @@ def test_hexagonal_supercell_against_lattice_enumeration(tmp_path):
     ga, nn = got[sp.supercell_sites["atom_type"] == "Ga"], got[sp.supercell_sites["atom_type"] == "N"]
     a, c, u = 3.189, 5.185, 0.377                  # wurtzite bonds: u c along c, sqrt(a^2/3 + (1/2 - u)^2 c^2) basal
     assert abs(cKDTree(nn).query(ga)[0].min() - min(u * c, np.sqrt(a * a / 3 + ((0.5 - u) * c) ** 2))) < 1e-4
+
+
+def test_primitive_fcc_transform_closed_form():
+    """transform_unit_cell('F', primitive=True) on diamond Si: the rhombohedral
+    primitive cell, a/sqrt(2) edges at 60 degrees, volume a^3/4, two atoms at 0
+    and a/4 (1, 1, 1); the oracle's restatement agrees."""
+    a = 5.43094
+    sp = S.SupercellBuilder(SI)
+    b0, ops0, s0 = sp.basis.copy(), list(sp.sym_ops), sp.sites.copy()
+    sp.transform_unit_cell(centering_type="F", primitive=True)
+    sp.build_unit_cell()
+    np.testing.assert_allclose(sp.lattice_const, [a / np.sqrt(2)] * 3, rtol=1e-12)
+    np.testing.assert_allclose(sp.lattice_angles, [60.0] * 3, rtol=1e-12)
+    assert abs(abs(sp.lattice_volume) - a ** 3 / 4) < 1e-9
+    np.testing.assert_allclose(np.sort(_pos(sp.xyz_positions), axis=0), [[0, 0, 0], [a / 4] * 3], atol=1e-5)
+    basis, _, _, ops, sites = O.transform(b0, ops0, s0, centering="F", primitive=True)
+    _, cart, _ = O.unit_cell(basis, ops, sites)
+    np.testing.assert_allclose(cart, _pos(sp.xyz_positions), atol=1e-5)
