@@ -110,20 +110,30 @@ def test_thin_object_matches_oracle(gpu):
     assert one.shape == (N, N) and rel_l2(one, ref[1]) < 3e-6
 
 
-def test_cif_supercell_multislice(gpu):
-    """Structure front-end feeding the GPU path: Si.cif (192 operations) ->
-    unit cell -> a 22 x 22 x 16 Å [110]/[001] supercell -> MSAGPU slices and
-    CBEDs, against the oracle's slices and CBEDs for the same records."""
+@pytest.mark.parametrize("material", ["Si", "GaN"])
+def test_cif_supercell_multislice(gpu, material, tmp_path):
+    """Structure front-end feeding the GPU path: Si.cif (192 operations, a
+    22 x 22 x 16 Å [110]/[001] box) and hexagonal GaN (P6_3mc, 20 x 20 x 12 Å)
+    -> unit cell -> supercell -> MSAGPU slices and CBEDs, against the oracle's
+    slices and CBEDs for the same records."""
     import os
 
     from namsa_amd.supercell import SupercellBuilder
     from tests.parity_util import oracle_pm, oracle_probe, oracle_slices
+    from tests.test_supercell import GAN_CIF
 
-    _say("Si.cif -> SupercellBuilder -> MSAGPU, 256^2")
-    sp = SupercellBuilder(os.path.join(os.path.dirname(__file__), "golden", "cif_files", "Si.cif"))
-    sp.build_unit_cell()
-    sp.make_orthogonal_supercell(supercell_size=np.array([22.0, 22.0, 16.0]), projec_1=np.array([1, 1, 0]),
-                                 projec_2=np.array([0, 0, 1]))
+    _say(f"{material} CIF -> SupercellBuilder -> MSAGPU, 256^2")
+    if material == "Si":
+        sp = SupercellBuilder(os.path.join(os.path.dirname(__file__), "golden", "cif_files", "Si.cif"))
+        sp.build_unit_cell()
+        sp.make_orthogonal_supercell(supercell_size=np.array([22.0, 22.0, 16.0]), projec_1=np.array([1, 1, 0]),
+                                     projec_2=np.array([0, 0, 1]))
+    else:
+        path = tmp_path / "gan.cif"
+        path.write_text(GAN_CIF)
+        sp = SupercellBuilder(str(path))
+        sp.build_unit_cell()
+        sp.make_orthogonal_supercell(supercell_size=np.array([20.0, 20.0, 12.0]))
     sim = MSAGPU(200, 0.025, sp.supercell_sites, sampling=np.array([256, 256]), debye_waller=False)
     sim.setup_device(0)
     sim.calc_atomic_potentials()
