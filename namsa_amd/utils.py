@@ -1,4 +1,6 @@
-"""Helpers with counterparts in the reference's namsa/utils.py, run on the GPU."""
+"""Helpers with counterparts in the reference's namsa/utils.py: the
+thin-object approximation (on the GPU) and the host-side crystallography and
+array helpers the reference's scripts use."""
 from __future__ import annotations
 
 import numpy as np
@@ -36,3 +38,35 @@ def thin_object_approx(potential_slices, probes, device=None):
     call("namsa_thin_object", ptr(V_d), V.shape[0], ptr(P_d), B, Ny, Nx, ptr(ws), ptr(out), stream_handle())
     res = out.cpu().numpy()
     return res[0] if single else res
+
+
+def dhkl_spacing(uc_volume, basis, hkl=(0, 0, 1)):
+    """Interplanar spacing 1 / |h b1 + k b2 + l b3| (Angstrom) from the lattice
+    vectors a1, a2, a3 (rows of `basis`) and the cell volume
+    (namsa/utils.py:91-98)."""
+    a1, a2, a3 = np.asarray(basis, dtype=np.float64)
+    recip = np.array([np.cross(a2, a3), np.cross(a3, a1), np.cross(a1, a2)]) / uc_volume
+    g = hkl[0] * recip[0] + hkl[1] * recip[1] + hkl[2] * recip[2]
+    return 1.0 / np.sqrt(np.dot(g, g))
+
+
+def bin_2d_array(arr, dwnspl=2, mode="mean"):
+    """Block mean (or, with any other mode, block sum) over dwnspl x dwnspl
+    blocks (namsa/utils.py:101-108); trailing rows/columns must divide."""
+    a = np.asarray(arr)
+    r, c = a.shape[0] // dwnspl, a.shape[1] // dwnspl
+    blocks = a.reshape(r, dwnspl, c, dwnspl)
+    return blocks.mean(-1).mean(-2) if mode == "mean" else blocks.sum(-1).sum(-2)
+
+
+def imageTile(data, padsize=1, padval=0):
+    """Tile [n][h][w...] images into one square mosaic, each image followed by
+    padsize rows and columns of padval (namsa/utils.py:111-127)."""
+    data = np.asarray(data)
+    n = int(np.ceil(np.sqrt(data.shape[0])))
+    pad = ((0, n * n - data.shape[0]), (0, padsize), (0, padsize)) + ((0, 0),) * (data.ndim - 3)
+    data = np.pad(data, pad, mode="constant", constant_values=(padval, padval))
+    h, w = data.shape[1], data.shape[2]
+    grid = data.reshape((n, n) + data.shape[1:])
+    grid = np.swapaxes(grid, 1, 2)                       # (row of tiles, h, column of tiles, w, ...)
+    return grid.reshape((n * h, n * w) + data.shape[3:])
