@@ -42,8 +42,8 @@ def process_cbed(cbed, normalize=True, scale=(-1, 1), fp16=False, new_shape=None
     deviation and return float16 (utils.py:104-118). The reference's keyword
     arguments are accepted and, as there, unused.
 
-    cbed: numpy array or torch tensor (a CUDA tensor is encoded in place on
-    its device). moments: the [P][2] (sum, sum of squares) of the same
+    cbed: numpy array or torch tensor (a CUDA tensor is read where it lies,
+    without a host copy; the input is not modified). moments: the [P][2] (sum, sum of squares) of the same
     CBEDs, as MSAGPU.multislice(moments=True) keeps them; then the encoder
     reads the CBEDs once instead of twice. Returns numpy float16, or the
     device tensor with as_tensor=True.
