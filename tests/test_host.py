@@ -148,3 +148,16 @@ def test_probe_group_policy(monkeypatch, tmp_path):
     assert msa.cache_group(512, 512) == 160 and msa.cache_group(2048, 2048) == 6
     assert msa.cache_group(256, 512) == int(llc * 0.75 // (8 * 256 * 512))
     assert msa.cache_group(8192, 8192) == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py reports one MI355X; --gpus above 1 exits with a message, before any GPU work."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parent.parent
+    res = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                         timeout=120, cwd=root)
+    assert res.returncode != 0
+    assert "one MI355X" in (res.stdout + res.stderr) and res.stdout.strip() == ""
