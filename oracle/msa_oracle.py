@@ -30,6 +30,15 @@ Pinning (tests/test_oracle.py):
   bin_sites / splat_slices ......... brute force one-atom placement, edge
                                       clipping, linearity, sentinel skip
   transmission ..................... |t| = 1, NaN -> 1 special case
+  geometry ......................... config KATs (64 A at 512^2: kmax 8.0,
+                                      pixel 0.125, max angle 200.6 mrad;
+                                      38 A at 256^2: 6.74, 0.148) and a
+                                      hand-computed anisotropic (py, px) case
+  aperture ......................... A = 1/2 exactly at float32 k = k_semi,
+                                      1/4 and 3/4 at ln(3)/160 either side
+                                      (steepness 2*80), monotone decay
+  chi_phase ........................ C1-, C3-, C5-only and combined closed
+                                      forms at hand-computed pixel radii
   k_axis / probe_k ................. closed-form spot values, |psi_k| = A
   probe_real / probe_stack ......... Parseval (norm = N^2 sum |psi_k|^2),
                                       position-0 stack == build_probe psi,
@@ -53,7 +62,7 @@ import math
 
 import numpy as np
 from scipy.constants import elementary_charge, electron_mass, speed_of_light
-from scipy.special import k0
+from scipy.special import expit, k0
 
 A0_BOHR = 0.529       # Angstrom, msa.py:183
 E_CHARGE = 14.4       # V*Angstrom, msa.py:184
@@ -229,7 +238,7 @@ def aperture(Ny, Nx, kmax_f32, k_semi_f32, smooth=True, steepness=80.0):
     kf = k_radius(Ny, Nx, kmax_f32).astype(np.float32).astype(np.float64)
     ks = np.float64(np.float32(k_semi_f32))
     if smooth:
-        return 1.0 / (1.0 + np.exp(-2.0 * steepness * (ks - kf)))
+        return expit(2.0 * steepness * (ks - kf))   # 1/(1+exp(-x)) without overflow far outside
     return (kf < ks).astype(np.float64)
 
 

@@ -162,6 +162,83 @@ def test_probe_amplitude_is_aperture_and_defocus_is_propagation():
     np.testing.assert_allclose(psi_k / A, np.conj(P), rtol=0, atol=2e-5)
 
 
+def test_soft_aperture_half_point_steepness_and_monotone_decay():
+    # probe_kernels.cu:196-206: A = 1/(1 + exp(-2*80*(k_semi - k))) on the float32 k radius
+    N, km = 64, 7.0
+    kf = O.k_radius(N, N, km).astype(np.float32)
+    j, i = 20, 45                                   # any pixel: put the aperture edge exactly there
+    ks = np.float64(kf[j, i])
+    A = O.aperture(N, N, km, ks, smooth=True)
+    assert A[j, i] == 0.5                            # exactly one half where float32 k == k_semi
+    # steepness 2*80 per inverse Angstrom: 1/4 of the way to 1 at ln(3)/160 inside the edge
+    A = O.aperture(N, N, km, ks + np.log(3.0) / 160.0, smooth=True)
+    assert A[j, i] == pytest.approx(0.75, abs=2e-5)   # float32 rounding of k_semi: 160 * 4.8e-7
+    A = O.aperture(N, N, km, ks - np.log(3.0) / 160.0, smooth=True)
+    assert A[j, i] == pytest.approx(0.25, abs=2e-5)
+    # non-increasing in k, 1 at the centre, 0 far outside; no overflow warning at the grid corner
+    order = np.argsort(kf.ravel(), kind="stable")
+    a = O.aperture(N, N, km, 1.0, smooth=True).ravel()[order]
+    assert np.all(np.diff(a) <= 0)
+    assert a[0] == pytest.approx(1.0, abs=1e-60) and a[-1] < 1e-200
+    # the steepness is a parameter: 2*s*(k_semi - k) in the exponent
+    A40 = O.aperture(N, N, km, ks + 0.01, smooth=True, steepness=40.0)
+    assert A40[j, i] == pytest.approx(1 / (1 + np.exp(-0.8)), abs=2e-5)
+
+
+def _pixel_k(N, km, j, i):
+    # |k| of pixel (row j, col i) on the linspace grid, written out by hand
+    ky = j * km / (N - 1) - km / 2
+    kx = i * km / (N - 1) - km / 2
+    return np.hypot(ky, kx)
+
+
+@pytest.mark.parametrize("C1,C3,C5,expect", [
+    (0.0, 1.2e7, 0.0, lambda k, lam: 2 * np.pi / lam * 1.2e7 * (k * lam) ** 4 / 4),
+    (0.0, 0.0, 3.0e9, lambda k, lam: 2 * np.pi / lam * 3.0e9 * (k * lam) ** 6 / 6),
+    (50.0, 0.0, 0.0, lambda k, lam: -np.pi * 50.0 * lam * k ** 2),
+    (50.0, 1.2e7, 3.0e9, lambda k, lam: np.pi / lam * (-50.0 * (k * lam) ** 2 + 1.2e7 * (k * lam) ** 4 / 2
+                                                        + 3.0e9 * (k * lam) ** 6 / 3)),
+])
+def test_chi_closed_forms(C1, C3, C5, expect):
+    # probe_kernels.cu:208-222: chi = 2 pi/lambda (-C1 (k lambda)^2/2 + C3 (k lambda)^4/4 + C5 (k lambda)^6/6);
+    # the kernel's float pi and float 1/6 differ from the exact constants by < 1e-7 relative
+    N, km, lam = 64, 8.0, np.float32(0.0196876)
+    chi = O.chi_phase(N, N, km, lam, C1, C3, C5)
+    for j, i in [(40, 40), (63, 0), (10, 52)]:
+        k = _pixel_k(N, np.float64(np.float32(km)), j, i)
+        assert chi[j, i] == pytest.approx(expect(k, np.float64(lam)), rel=2e-7)
+
+
+@pytest.mark.parametrize("lateral,N,kv,kmax,pix,max_ang", [
+    (64.0, 512, 200.0, 8.0, 0.125, 0.200636),      # config 2: 64 x 64 A at 512^2, 200 kV
+    (38.0, 256, 100.0, 6.736842, 0.1484375, 0.249361),   # config 1 as specified: ~38 x 38 A at 256^2, 100 kV
+])
+def test_geometry_config_kats(lateral, N, kv, kmax, pix, max_ang):
+    # msa.py:110-123 on a box whose atoms span exactly lateral x lateral x 100 A
+    xyz = np.array([[0.0, 0.0, 0.0], [lateral, lateral, 100.0], [lateral / 2, 3.0, 50.0]], np.float32)
+    g = O.geometry(xyz, (N, N), kv)
+    assert g["kmax"] == pytest.approx(kmax, rel=1e-6)
+    np.testing.assert_allclose(g["pix_size"], [pix, pix], rtol=1e-6)
+    assert g["max_ang"] == pytest.approx(max_ang, rel=1e-5)
+    np.testing.assert_allclose(g["kpix_size"], [kmax / N, kmax / N], rtol=1e-6)
+
+
+def test_geometry_anisotropic_known_answer():
+    # atoms spanning x 0..30, y 0..20, z 0..10 A on a (Ny, Nx) = (128, 256) grid:
+    # kmax = min(sampling / dims[:2]) = min(128/30, 256/20)  (sampling[0] meets the x extent, msa.py:114)
+    # pix_size = dims[:2][::-1] / sampling = (20/128, 30/256) = (py, px)  (msa.py:121)
+    xyz = np.array([[0.0, 0.0, 0.0], [30.0, 20.0, 10.0], [12.0, 7.0, 3.0]], np.float32)
+    g = O.geometry(xyz, (128, 256), 300.0)
+    np.testing.assert_array_equal(g["dims"], np.array([30.0, 20.0, 10.0], np.float32))
+    assert g["kmax"] == 128 / 30
+    np.testing.assert_array_equal(g["pix_size"], [20 / 128, 30 / 256])
+    np.testing.assert_array_equal(g["kpix_size"], [128 / 30 / 128, 128 / 30 / 256])
+    assert g["max_ang"] == pytest.approx(128 / 30 * 0.0196876, rel=2e-6)
+    # negative coordinates: only the extents matter
+    g2 = O.geometry(xyz - np.float32(5.0), (128, 256), 300.0)
+    np.testing.assert_allclose(g2["pix_size"], g["pix_size"], rtol=1e-6)
+
+
 def test_hard_aperture_and_mask_are_float32_comparisons():
     N, km = 32, 7.0
     k32 = O.k_radius(N, N, km).astype(np.float32)
