@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import NamsaError, call, ptr, stream_handle
-from .optics import scherzer_params, sigma_int, voltage2Lambda
+from .optics import scherzer_defocus, sigma_int, voltage2Lambda
 
 XYZ_dtype = [("atomic_number", "i"), ("x", "f"), ("y", "f"), ("z", "f"), ("occ", "f"), ("DW", "f")]
 KIRKLAND_PATH = Path(__file__).resolve().parent / "data" / "kirkland_params.npy"
@@ -75,58 +75,96 @@ def cache_group(Ny, Nx):
     return max(1, int(LLC_BYTES * share // (8 * Ny * Nx)))
 
 
-class _ChunkStreamer:
-    """Copies finished chunks of the CBED output to pinned host memory on a
-    second stream (default priority) and hands them to a sink, one chunk
-    behind the compute stream, so copies and the sink overlap the next
-    chunk's slices. With moments, each chunk is first encoded into float16
-    records on the compute stream (namsa_cbed_records_from_moments)."""
+def phase_plan(P, max_probes, group, sink=False, batch_size=None):
+    """How multislice splits P probe positions: (chunk, resident).
 
-    def __init__(self, out, moments, chunk, sink):
+    chunk: probes per call of the fused driver, at most ``max_probes`` (the
+    memory planner's bound, msa.py:445-461) and ``batch_size``; with a sink
+    and no batch_size, a multiple of the probe group giving about eight
+    chunks, so copies overlap compute.  resident: the whole output stays on
+    the device (no sink and every position fits at once); otherwise the run
+    goes in phases of ``chunk`` positions, in order (the reference's phases,
+    msa.py:843-907), with device memory for two chunks of output only.
+    """
+    P, group = int(P), max(1, int(group))
+    chunk = max(1, min(int(max_probes), P))
+    if sink and not batch_size:
+        chunk = min(chunk, group * max(1, -(-P // group) // 8))
+    if batch_size:
+        chunk = min(chunk, int(batch_size))
+    return chunk, (not sink) and P <= int(max_probes)
+
+
+class _ChunkStreamer:
+    """Moves finished chunks of a phased multislice off the device.
+
+    The driver writes chunk k into device slot k % n_slots; this object copies
+    it on a second stream (default priority) to pinned host memory -- the
+    caller's full host output, or double-buffered staging handed to a sink one
+    chunk behind the compute stream -- so the copy and the sink overlap the
+    next chunk's slices.  With moments, each chunk is first encoded into
+    float16 records on the compute stream (namsa_cbed_records_from_moments).
+    The compute stream waits for a slot's copy before writing into it again.
+    """
+
+    def __init__(self, Ny, Nx, dtype, device, chunk, n_slots, sink=None, host_out=None, moments=None):
         torch = _torch()
-        self.out, self.moments, self.sink = out, moments, sink
-        P, Ny, Nx = (int(v) for v in out.shape)
-        self.Ny, self.Nx = Ny, Nx
-        dtype = torch.float16 if moments is not None else torch.float32
-        self.host = [torch.empty((chunk, Ny, Nx), dtype=dtype, pin_memory=True) for _ in range(2)]
-        self.enc = ([torch.empty((chunk, Ny, Nx), dtype=torch.float16, device=out.device) for _ in range(2)]
+        self.Ny, self.Nx, self.sink, self.host_out, self.moments = Ny, Nx, sink, host_out, moments
+        stage = torch.float16 if moments is not None else dtype
+        self.host = ([torch.empty((chunk, Ny, Nx), dtype=stage, pin_memory=True) for _ in range(2)]
+                     if sink is not None else None)
+        self.enc = ([torch.empty((chunk, Ny, Nx), dtype=torch.float16, device=device) for _ in range(n_slots)]
                     if moments is not None else None)
-        self.copy_stream = torch.cuda.Stream(device=out.device)
-        self.done = [None, None]
+        self.copy_stream = torch.cuda.Stream(device=device)
+        self.copied = [None] * n_slots      # event: the slot's device data has left the device
+        self.done = [None, None]            # event: the staging slot's copy finished
         self.pending = None
         self.k = 0
 
-    def push(self, start, B):
+    def claim(self, slot):
+        """Make the compute stream wait until device slot `slot` may be overwritten."""
+        if self.copied[slot] is not None:
+            _torch().cuda.current_stream().wait_event(self.copied[slot])
+
+    def push(self, src, start, B, slot):
         torch = _torch()
-        slot = self.k % 2
-        self.k += 1
-        if self.done[slot] is not None:          # the slot's previous copy and sink have finished
-            self.done[slot].synchronize()
-        src = self.out[start:start + B]
         if self.enc is not None:
             call("namsa_cbed_records_from_moments", ptr(src), ptr(self.moments[start]), B, self.Ny, self.Nx,
                  ptr(self.enc[slot]), stream_handle())
             src = self.enc[slot][:B]
         ready = torch.cuda.Event()
-        ready.record(torch.cuda.current_stream(self.out.device))
+        ready.record(torch.cuda.current_stream(src.device))
         self.copy_stream.wait_event(ready)
+        if self.sink is None:
+            with torch.cuda.stream(self.copy_stream):
+                self.host_out[start:start + B].copy_(src, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.copy_stream)
+            self.copied[slot] = ev
+            return
+        hs = self.k % 2
+        self.k += 1
+        if self.done[hs] is not None:      # the staging slot's previous chunk was delivered already
+            self.done[hs].synchronize()
         with torch.cuda.stream(self.copy_stream):
-            self.host[slot][:B].copy_(src, non_blocking=True)
+            self.host[hs][:B].copy_(src, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(self.copy_stream)
-        self.done[slot] = ev
+        self.copied[slot] = ev
+        self.done[hs] = ev
         if self.pending is not None:
             self._deliver(*self.pending)
-        self.pending = (slot, start, B)
+        self.pending = (hs, start, B)
 
-    def _deliver(self, slot, start, B):
-        self.done[slot].synchronize()
-        self.sink(start, self.host[slot][:B].numpy())
+    def _deliver(self, hs, start, B):
+        self.done[hs].synchronize()
+        self.sink(start, self.host[hs][:B].numpy())
 
     def finish(self):
         if self.pending is not None:
             self._deliver(*self.pending)
             self.pending = None
+        self.copy_stream.synchronize()
 
 
 class MSAGPU:
@@ -372,7 +410,7 @@ class MSAGPU:
         aber = probe_dict.get("aberration_dict", {"C1": 0.0, "C3": 0.0, "C5": 0.0})
         C1, C3, C5 = (np.float32(aber.get(k, 0.0)) for k in ("C1", "C3", "C5"))
         if probe_dict.get("scherzer", False):  # GPU rule: always replaces C1 (msa.py:677-679)
-            C1 = np.float32(scherzer_params(self.Lambda, aber.get("C3", 0.0))[0])
+            C1 = np.float32(scherzer_defocus(self.Lambda, aber.get("C3", 0.0)))
         if not probe_dict.get("spherical_phase", True):
             C1 = C3 = C5 = np.float32(0.0)
         return C1, C3, C5
@@ -432,18 +470,31 @@ class MSAGPU:
         self.grid_range = np.array([rx, ry]).flatten()
         self.num_probes = np.int32(self.probe_positions.shape[0])
 
-    def plan_simulation(self, num_probes=None):
-        """Largest probe batch whose buffers fit half the free memory (msa.py:445-461)."""
+    def plan_simulation(self, num_probes=None, group=None):
+        """Largest number of probe positions whose device buffers fit half the
+        free memory (msa.py:445-461).
+
+        Per position: 16 N^2 bytes, the worst case of the output (two
+        double-buffered complex64 chunk planes when the run goes in phases);
+        plus one probe group of wave state and the resident slices.  As in the
+        reference, the count shrinks by 3/4 until that fits.  ``multislice``
+        runs more positions than this in phases (``phase_plan``).
+        """
         torch = _torch()
         self._dev()
         if num_probes is None:
             num_probes = int(self.num_probes)
         free, _ = torch.cuda.mem_get_info(self.device)
-        plane = int(np.prod(self.sampling))
-        slices = 0 if self._t is None else self._t.numel() * 8
-        per_probe = plane * (8 + 4)  # wave state + float32 CBED
+        Ny, Nx = self._grid
+        plane = Ny * Nx
+        slices = 0 if self._t is None else int(self._t.numel()) * 8
+        G = int(group) if group else cache_group(Ny, Nx)
+
+        def need(n):
+            return n * 16 * plane + min(n, G) * 8 * plane + slices
+
         n = int(num_probes)
-        while n > 1 and n * per_probe + slices > 0.5 * free:
+        while n > 1 and need(n) > 0.5 * free:
             n = int(n * 0.75)
         self.max_probes = max(1, n)
         self.print_verbose("Device can hold at most %d probes simultaneously." % self.max_probes)
@@ -483,6 +534,16 @@ class MSAGPU:
         readout, in ``cbed_moments_device`` [P][2] float64, for
         ``records.process_cbed(..., moments=...)``.
 
+        Memory: when every position fits at once (``plan_simulation``'s
+        ``max_probes``) and there is no sink, the output stays on the device
+        (``probes_device``; ``probes`` copies it to host on first use).
+        Otherwise the positions go in phases of at most ``max_probes``
+        (``batch_size`` if smaller), in order, as the reference's phases do
+        (msa.py:843-907): the device holds two chunks of output, and each
+        finished chunk is copied on a second stream to pinned host memory
+        while the next one computes -- into ``probes`` (a pinned [P][Ny][Nx]
+        host array), or to a sink.  ``probes_device`` is then None.
+
         sink(start, array): stream the CBEDs out while the simulation runs.
         The probes go in chunks (``batch_size``, by default a multiple of
         the group giving about eight chunks). Each chunk's CBEDs, or with
@@ -490,7 +551,8 @@ class MSAGPU:
         readout's moments), are copied to pinned host memory on a second
         stream, while the next chunk computes. sink then gets the chunk's
         first probe index and a host array, which is only valid during the
-        call. The device CBEDs stay available as ``probes_device``.
+        call. Nothing else keeps the CBEDs: ``probes`` and ``probes_device``
+        are None.
         """
         torch = _torch()
         dev = self._dev()
@@ -504,32 +566,28 @@ class MSAGPU:
         if getattr(self, "probe_positions", None) is None:
             warn("Probe positions must be calculated first before calling multislice.")
             return None
-        if not hasattr(self, "max_probes"):
-            self.plan_simulation()
         Ny, Nx = self._grid
         P = int(self.num_probes)
+        group = int(group) if group else cache_group(Ny, Nx)
+        if not hasattr(self, "max_probes"):
+            self.plan_simulation(group=group)
         S = int(self.num_slices) if transmit else 0
         self.build_propagator(bandwidth)
         mode = _lib.OUT_CBED if transmit else _lib.OUT_INCIDENT
         if output == "exit_wave":
             mode = _lib.OUT_EXIT
             S = int(self.num_slices)
-        chunk = min(int(self.max_probes), P)
-        group = int(group) if group else cache_group(Ny, Nx)
         if sink is not None and mode != _lib.OUT_CBED:
             raise NamsaError("sink streams CBEDs: it needs transmit=True and CBED output")
-        if sink is not None and not batch_size:
-            chunk = min(chunk, group * max(1, -(-P // group) // 8))
-        if batch_size:
-            chunk = min(chunk, int(batch_size))
+        chunk, resident = phase_plan(P, self.max_probes, group, sink is not None, batch_size)
         ws_planes = min(chunk, group)
         pos_all = torch.from_numpy(np.ascontiguousarray(self.probe_positions, dtype=np.float32)).to(dev)
         if mode == _lib.OUT_CBED:
+            odt = torch.float32
             psi_ws = torch.empty((ws_planes, Ny, Nx), dtype=torch.complex64, device=dev)
-            out = torch.empty((P, Ny, Nx), dtype=torch.float32, device=dev)
             scale = np.float32(1.0 / self.normalization)
-        else:
-            out = torch.empty((P, Ny, Nx), dtype=torch.complex64, device=dev)
+        else:   # the wave state itself is the output
+            odt = torch.complex64
             scale = np.float32(1.0 / np.sqrt(self.normalization))
         det = None
         nDet = 0
@@ -548,14 +606,29 @@ class MSAGPU:
             self._moments_dev = torch.empty((P, 2), dtype=torch.float64, device=dev)
         st = stream_handle()
         t_arg = ptr(self._t) if S > 0 else None
-        if sink is not None:
-            streamer = _ChunkStreamer(out, self._moments_dev if records else None, chunk, sink)
-        for start in range(0, P, chunk):
+        self._probes_dev = None
+        self._probes_host = None
+        self._host_pinned = None
+        if resident:
+            out = torch.empty((P, Ny, Nx), dtype=odt, device=dev)
+        else:
+            slots = [torch.empty((chunk, Ny, Nx), dtype=odt, device=dev) for _ in range(2 if P > chunk else 1)]
+            if sink is None:
+                self._host_pinned = torch.empty((P, Ny, Nx), dtype=odt, pin_memory=True)
+            streamer = _ChunkStreamer(Ny, Nx, odt, dev, chunk, len(slots), sink=sink, host_out=self._host_pinned,
+                                      moments=self._moments_dev if (sink is not None and records) else None)
+        for k, start in enumerate(range(0, P, chunk)):
             B = min(chunk, P - start)
+            if resident:
+                dst = out[start:start + B]
+            else:
+                slot = k % len(slots)
+                streamer.claim(slot)
+                dst = slots[slot][:B]
             if mode == _lib.OUT_CBED:
-                cbed_ptr, ws_ptr = ptr(out[start]), ptr(psi_ws)
-            else:  # the wave state itself is the output
-                cbed_ptr, ws_ptr = None, ptr(out[start])
+                cbed_ptr, ws_ptr = ptr(dst), ptr(psi_ws)
+            else:
+                cbed_ptr, ws_ptr = None, ptr(dst)
             det_args = (ptr(det), nDet, ptr(det_ws), ptr(self._det_dev[start])) if nDet else (None, 0, None, None)
             if with_moments:
                 call("namsa_multislice_moments", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
@@ -565,19 +638,24 @@ class MSAGPU:
                 call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
                      np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
                      group, mode, scale, ws_ptr, cbed_ptr, *det_args, st)
-            if sink is not None:
-                streamer.push(start, B)
-        if sink is not None:
+            if not resident:
+                streamer.push(dst, start, B, slot)
+        if resident:
+            self._probes_dev = out
+        else:
             streamer.finish()
-        self._probes_dev = out
-        self._probes_host = None
+            if self._host_pinned is not None:
+                self._probes_host = self._host_pinned.numpy()
         self.detector_signals = self._det_dev.cpu().numpy() if nDet else None
         self.vars.append(self._t)
         return None
 
     @property
     def probes(self):
-        """Host copy of the multislice output (CBED float32 or complex probes)."""
+        """Host copy of the multislice output (CBED float32 or complex probes);
+        None after a run that streamed its output to a sink."""
+        if self._probes_host is not None:
+            return self._probes_host
         if self._probes_dev is None:
             return None
         if self._probes_host is None:
@@ -621,6 +699,24 @@ class MSAGPU:
             return ((theta >= inner_angle) & (theta < outer_angle)).astype(np.float32)
         raise ValueError("mode must be 'reference' or 'physical'")
 
+    def _cbed_blocks(self):
+        """(start, device float32 CBEDs) blocks of the last run's output: the
+        resident device output in one block, or a phased run's pinned host
+        output uploaded one chunk at a time."""
+        torch = _torch()
+        if self._probes_dev is not None:
+            if self._probes_dev.dtype != torch.float32:
+                raise NamsaError("integrate_cbed needs CBEDs from multislice(transmit=True)")
+            yield 0, self._probes_dev
+            return
+        host = getattr(self, "_host_pinned", None)
+        if host is None or host.dtype != torch.float32:
+            raise NamsaError("integrate_cbed needs CBEDs from multislice(transmit=True) kept in probes "
+                             "(a run that streamed them to a sink keeps none)")
+        step = max(1, min(int(getattr(self, "max_probes", host.shape[0])), host.shape[0]))
+        for start in range(0, host.shape[0], step):
+            yield start, host[start:start + step].to(self.device, non_blocking=True)
+
     def integrate_cbed(self, detector_array=None,
                        detector_params={"inner_angle": 50e-3, "outer_angle": 100e-3}):
         """Detector signal per raster point, shaped grid_steps (msa.py:426-439)."""
@@ -631,31 +727,31 @@ class MSAGPU:
                                                 detector_params["outer_angle"])
         Ny, Nx = self._grid
         det = torch.as_tensor(np.asarray(detector_array, dtype=np.float32).reshape(1, Ny, Nx)).to(dev)
-        cb = self._probes_dev
-        if cb is None or cb.dtype != torch.float32:
-            raise NamsaError("integrate_cbed needs CBEDs from multislice(transmit=True)")
-        P = cb.shape[0]
-        out = torch.empty((P, 1), dtype=torch.float64, device=dev)
-        call("namsa_integrate", ptr(cb), P, Ny, Nx, ptr(det), 1, ptr(out), stream_handle())
+        out = torch.empty((int(self.num_probes), 1), dtype=torch.float64, device=dev)
+        for start, cb in self._cbed_blocks():
+            call("namsa_integrate", ptr(cb), cb.shape[0], Ny, Nx, ptr(det), 1, ptr(out[start]), stream_handle())
         signal = out[:, 0].cpu().numpy().reshape([int(g) for g in self.grid_steps])
         return signal, np.asarray(detector_array)
 
     def check_simulation(self):
         """Integrated intensity per probe; warns outside [0.95, 1.0] (msa.py:363-371)."""
-        cb = self._probes_dev
-        if cb is None:
-            return None
         torch = _torch()
-        P = cb.shape[0]
         Ny, Nx = self._grid
-        out = torch.empty((P,), dtype=torch.float64, device=cb.device)
         st = stream_handle()
-        if cb.dtype == torch.float32:
-            ones = torch.ones((1, Ny, Nx), dtype=torch.float32, device=cb.device)
-            call("namsa_integrate", ptr(cb), P, Ny, Nx, ptr(ones), 1, ptr(out), st)
-        else:
-            for i in range(P):
+        cb = self._probes_dev
+        if cb is not None and cb.dtype != torch.float32:
+            out = torch.empty((cb.shape[0],), dtype=torch.float64, device=cb.device)
+            for i in range(cb.shape[0]):
                 call("namsa_norm2", ptr(cb[i]), Ny * Nx, ptr(out[i]), st)
+        elif cb is not None or getattr(self, "_host_pinned", None) is not None:
+            if cb is None and self._host_pinned.dtype != torch.float32:
+                raise NamsaError("check_simulation of phased complex outputs is not supported")
+            out = torch.empty((int(self.num_probes),), dtype=torch.float64, device=self.device)
+            ones = torch.ones((1, Ny, Nx), dtype=torch.float32, device=self.device)
+            for start, blk in self._cbed_blocks():
+                call("namsa_integrate", ptr(blk), blk.shape[0], Ny, Nx, ptr(ones), 1, ptr(out[start]), st)
+        else:
+            return None
         prob = out.cpu().numpy()
         print("Max (Min) Integrated Intensity: %2.2f (%2.2f)" % (prob.max(), prob.min()))
         if prob.max() > 1.0 or prob.min() < 0.95:

@@ -25,13 +25,19 @@ def sigma_int(voltage):
     return 2 * np.pi * (_MC2 + e_v) / (2 * _MC2 + e_v) / (voltage2Lambda(voltage) * voltage)
 
 
+def scherzer_defocus(Lambda, Cs):
+    """Scherzer defocus sqrt(1.5 Cs lambda) alone: the defocus term of
+    scherzer_params, without its aperture (which divides by Cs)."""
+    return np.sqrt(1.5) * np.sqrt(Cs * Lambda)
+
+
 def scherzer_params(Lambda, Cs):
     """(defocus, aperture) at Scherzer conditions.
 
     The aperture keeps the reference's operator precedence, (6*lambda/Cs)**1/4
     == (6*lambda/Cs)/4; callers only use the defocus.
     """
-    defocus = np.sqrt(1.5) * np.sqrt(Cs * Lambda)
+    defocus = scherzer_defocus(Lambda, Cs)
     aperture = (6 * Lambda / Cs) ** 1 / 4
     return defocus, aperture
 

@@ -163,7 +163,8 @@ def test_records_from_readout_moments(gpu, N, bw):
 def test_streaming_sink(gpu, records):
     """multislice(sink=...): chunks of 8 probes reach the sink in order, one
     behind the compute stream. Their CBEDs (or float16 records) are
-    bit-identical to the device results of the same run."""
+    bit-identical to the device results of a run without a sink; the sink
+    run keeps no copy of its own."""
     from namsa_amd import DEFAULT_PROBE, MSAGPU
     from tests.parity_util import random_block
 
@@ -175,16 +176,54 @@ def test_streaming_sink(gpu, records):
     sim.build_probe(dict(DEFAULT_PROBE))
     sim.generate_probe_positions(grid_steps=np.array([6, 6]))
     sim.plan_simulation()
+    sim.multislice(bandwidth=2.0 / 3, moments=True)
+    ref = sim.probes.copy()
+    want = REC.process_cbed(sim.probes_device, moments=sim.cbed_moments_device)
     got = []
     sim.multislice(bandwidth=2.0 / 3, batch_size=8, sink=lambda s, a: got.append((s, a.copy())), records=records)
+    assert sim.probes is None and sim.probes_device is None
     assert [s for s, _ in got] == [0, 8, 16, 24, 32] and [len(a) for _, a in got] == [8, 8, 8, 8, 4]
     streamed = np.concatenate([a for _, a in got])
     if records:
-        want = REC.process_cbed(sim.probes_device, moments=sim.cbed_moments_device)
         assert streamed.dtype == np.float16
         assert np.array_equal(streamed.view(np.uint16), want.view(np.uint16))
-        ex, ms = _exact_cbed(sim.probes)
-        _check(streamed, R.process_cbed(sim.probes), "streamed records", ex, ms)
+        ex, ms = _exact_cbed(ref)
+        _check(streamed, R.process_cbed(ref), "streamed records", ex, ms)
     else:
         assert streamed.dtype == np.float32
-        assert np.array_equal(streamed.view(np.uint32), sim.probes.view(np.uint32))
+        assert np.array_equal(streamed.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("output", ["cbed", "exit_wave"])
+def test_phased_run_matches_resident(gpu, output):
+    """More positions than max_probes: the run goes in phases, in order, with
+    two chunks of output on the device, into pinned host memory.  The result
+    is bit-identical to a run that keeps everything on the device, and the
+    detector integrals of the host output equal the resident ones."""
+    from namsa_amd import DEFAULT_PROBE, MSAGPU
+    from tests.parity_util import random_block
+
+    _say(f"phased vs resident, 256^2, 30 positions in phases of 7, output={output}")
+    sim = MSAGPU(200, 0.025, random_block(23, 32.0, 12.0, 600), sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(2.0)
+    sim.build_probe(dict(DEFAULT_PROBE))
+    sim.generate_probe_positions(grid_steps=np.array([6, 5]))
+    sim.plan_simulation()
+    assert sim.max_probes >= 30
+    kw = dict(bandwidth=1 / 3, group=4, output=output)
+    sim.multislice(**kw)
+    assert sim.probes_device is not None
+    ref = sim.probes.copy()
+    if output == "cbed":
+        ref_sig, _ = sim.integrate_cbed(detector_params={"inner_angle": 10e-3, "outer_angle": 60e-3})
+    sim.max_probes = 7
+    sim.multislice(**kw)
+    assert sim.probes_device is None and sim.probes.shape == ref.shape
+    assert np.array_equal(sim.probes.view(np.uint32), ref.view(np.uint32))
+    if output == "cbed":
+        sig, _ = sim.integrate_cbed(detector_params={"inner_angle": 10e-3, "outer_angle": 60e-3})
+        np.testing.assert_array_equal(sig, ref_sig)
+        prob = sim.check_simulation()
+        assert prob.shape == (30,) and (prob > 0.9).all() and (prob < 1.0 + 1e-5).all()

@@ -161,3 +161,64 @@ def test_bench_refuses_more_than_one_gpu():
                          timeout=120, cwd=root)
     assert res.returncode != 0
     assert "one MI355X" in (res.stdout + res.stderr) and res.stdout.strip() == ""
+
+
+class _FakeSlices:
+    """Stands in for the device slices: plan_simulation only reads numel()."""
+
+    def __init__(self, S, N):
+        self.n = S * N * N
+
+    def numel(self):
+        return self.n
+
+
+@pytest.mark.parametrize("N,P,S,free_gib,expect_max", [
+    (512, 4096, 150, 280.0, 4096),      # config 2 fits at once: the output stays on the device
+    (2048, 4096, 1000, 280.0, 1728),    # config 5: 4096 -> 3072 -> 2304 -> 1728 (x3/4 each), 149.7 of 150.3 GB
+    (512, 4096, 150, 4.0, 306),         # a small card (4 GiB free): phases of 306
+])
+def test_memory_plan_and_phases_with_faked_free_memory(monkeypatch, tmp_path, N, P, S, free_gib, expect_max):
+    """plan_simulation sizes the phases from torch.cuda.mem_get_info as the
+    reference sizes them from cuda.mem_get_info (msa.py:445-461): 16 N^2
+    bytes per position, one probe group of state and the resident slices
+    within half the free memory, shrinking by 3/4.  phase_plan then covers
+    every position once, in order, never more than max_probes at a time."""
+    import torch
+
+    from namsa_amd import msa
+
+    monkeypatch.setenv("NAMSA_TUNE_CACHE", str(tmp_path / "none.json"))
+    free = int(free_gib * (1 << 30))
+    monkeypatch.setattr(torch.cuda, "mem_get_info", lambda device=None: (free, 288 << 30))
+    xyz = np.array([[0.0, 0.0, 0.0], [64.0, 64.0, 300.0], [10.0, 20.0, 30.0]])
+    sim = _sim(structures.make_records(xyz, np.array([14, 14, 79])), N=N)
+    sim.device = torch.device("cpu")          # host-only: no device is touched
+    sim._t = _FakeSlices(S, N)
+    sim.num_probes = np.int32(P)
+    G = msa.cache_group(N, N)
+    got = sim.plan_simulation()
+    assert got == sim.max_probes == expect_max
+    plane = N * N
+    need = lambda n: n * 16 * plane + min(n, G) * 8 * plane + S * plane * 8
+    assert need(got) <= 0.5 * free
+    assert got == P or need(int(got / 0.75) + 1) > 0.5 * free   # the shrink stopped at the first fit
+    for sink, batch in ((False, None), (True, None), (False, 100)):
+        chunk, resident = msa.phase_plan(P, got, G, sink, batch)
+        assert 1 <= chunk <= got and (batch is None or chunk <= batch)
+        assert resident == (not sink and P <= got)
+        starts = list(range(0, P, chunk))
+        order = np.concatenate([np.arange(s, min(P, s + chunk)) for s in starts])
+        np.testing.assert_array_equal(order, np.arange(P))     # in order, each position once
+        if sink and batch is None:
+            assert chunk % G == 0 or chunk == got
+
+
+def test_phase_plan_small_cases():
+    from namsa_amd.msa import phase_plan
+
+    assert phase_plan(36, 1000, 160, sink=True, batch_size=8) == (8, False)
+    assert phase_plan(36, 1000, 160) == (36, True)
+    assert phase_plan(36, 10, 160) == (10, False)
+    assert phase_plan(4096, 4096, 160, sink=True) == (480, False)   # 160 * (26 // 8): ~8 chunks
+    assert phase_plan(1, 1, 6) == (1, True)
