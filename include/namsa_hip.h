@@ -9,7 +9,8 @@
  *     (float re, float im), C order;
  *   - `stream` is a hipStream_t passed as void* (NULL = default stream); every
  *     call only enqueues work on it and never synchronises the host;
- *   - no entry point allocates device memory; scratch space is passed in;
+ *   - no entry point allocates device memory; scratch space is passed in
+ *     (namsa_multislice_noalloc uses a fixed per-device array instead);
  *   - every call returns 0 on success or a negative status; the message of
  *     the last failure on the calling thread is namsa_last_error();
  *   - grid sizes Ny, Nx must be supported FFT lengths (namsa_fft_supported).
@@ -253,6 +254,30 @@ int namsa_multislice_moments(const void* psi_k, const float* pos_yx, int B, int 
                              const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
                              float out_scale, void* psi_ws, float* cbed, const float* det, int nDet,
                              double* det_ws, double* det_out, double* mom_ws, double* mom_out, void* stream);
+
+/*
+ * namsa_multislice in NAMSA_OUT_CBED mode that allocates nothing and takes no
+ * workspace: the wave state of one probe group and its detector partial sums
+ * live in a fixed device array of NAMSA_SCRATCH_BYTES that each GPU the
+ * library runs on holds until the process ends.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, serving loops with a fixed memory budget).
+ *   group: probes per group; 0 = as many as the scratch holds, i.e.
+ *          floor(NAMSA_SCRATCH_BYTES / (8 Ny Nx + 8 (Nx/4) nDet)) (31 at
+ *          512^2 with two detectors, 1 at 2048^2).  A larger explicit group,
+ *          or a grid of which one probe does not fit, is rejected
+ *          (NAMSA_ERR_ARG) before anything is enqueued.
+ * Other arguments and results as namsa_multislice.  Every call on one GPU
+ * shares that GPU's array, and calls that overlap on one GPU (two streams,
+ * two threads) race; the array keeps the last call's contents, none of which
+ * a call reads before writing it.  namsa_init looks up the array's address
+ * for its device; the first call on a device that skipped namsa_init does so.
+ */
+#define NAMSA_SCRATCH_BYTES (64u << 20)
+int namsa_multislice_noalloc(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                             const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                             float out_scale, float* cbed, const float* det, int nDet, double* det_out,
+                             void* stream);
 
 /*
  * Row transforms of planes in place: forward (unscaled, scale must be 1) or

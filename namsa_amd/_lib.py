@@ -44,6 +44,8 @@ SIGNATURES = {
                               _vp, _i, _vp, _vp, _vp]),
     "namsa_multislice_moments": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _i,
                                       _vp, _vp, _vp, _vp, _vp]),
+    "namsa_multislice_noalloc": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _i, _vp, _i, _i, _i, _f, _vp, _vp, _i,
+                                      _vp, _vp]),
     "namsa_sched_count": (_i, [_i, _i]),
     "namsa_sched_name": (C.c_char_p, [_i, _i, _i]),
     "namsa_sched_get": (_i, [_i, _i]),

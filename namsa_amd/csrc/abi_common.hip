@@ -57,7 +57,7 @@ int namsa_init(int device) {
   rc = namsa::check_hip(hipSetDevice(device), "hipSetDevice");
   if (rc) return rc;
   (void)namsa::device_cu_count();
-  return namsa::kOk;
+  return namsa::scratch_setup();
 }
 
 }  // extern "C"

@@ -21,6 +21,9 @@ int fail(int code, const char* fmt, ...);
 int check_hip(hipError_t err, const char* what);
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
+// Look up the current device's address of the fixed scratch array of
+// namsa_multislice_noalloc (once per device; namsa_init calls it).
+int scratch_setup();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -17,12 +17,48 @@
 // (msa.py:985-990) equals N^2 |PM * C(u_last)|^2 because FFT IFFT_u = N^2,
 // so the last column pass writes intensities instead of transforming back.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "namsa_hip.h"
 #include "passes.hpp"
 
 namespace namsa {
+
+// Fixed device memory of namsa_multislice_noalloc: the wave state of one
+// probe group followed by its detector partial sums.  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code the name
+// is only a stand-in for the symbol; kernels get the per-device address that
+// scratch_setup() looks up.
+static __device__ __align__(256) unsigned char scratch[NAMSA_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // indexed by device ordinal
+
+int scratch_setup() {
+  int dev = 0;
+  if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev >= static_cast<int>(g_scratch_ptr.size())) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (g_scratch_ptr[dev]) return kOk;
+  void* p = nullptr;
+  if (int rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+    return rc;
+  if (!p) return fail(kErrHip, "hipGetSymbolAddress(scratch) returned a null address");
+  g_scratch_ptr[dev] = p;
+  return kOk;
+}
+
+// Address of the current device's scratch array (looked up on first use).
+static int scratch_address(unsigned char** out) {
+  if (int rc = scratch_setup()) return rc;
+  int dev = 0;
+  if (int rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return kOk;
+}
 
 // det_out[b][d] = sum over the column tiles of image b, in tile order.
 __global__ void det_finalize_kernel(const double* __restrict__ ws, int tiles_per_image, int n_det,
@@ -421,6 +457,33 @@ int namsa_multislice_moments(const void* psi_k, const float* pos_yx, int B, int 
   if (!mom_ws || !mom_out) return fail(kErrArg, "namsa_multislice_moments: null moments buffers");
   return multislice_impl(psi_k, pos_yx, B, Ny, Nx, kmax, t, S, pm, band_lo, band_hi, group, NAMSA_OUT_CBED,
                          out_scale, psi_ws, cbed, det, nDet, det_ws, det_out, mom_ws, mom_out, stream);
+}
+
+int namsa_multislice_noalloc(const void* psi_k, const float* pos_yx, int B, int Ny, int Nx, float kmax,
+                             const void* t, int S, const void* pm, int band_lo, int band_hi, int group,
+                             float out_scale, float* cbed, const float* det, int nDet, double* det_out,
+                             void* stream) {
+  if (int rc = check_grid(Ny, Nx, "namsa_multislice_noalloc")) return rc;
+  if (B <= 0 || group < 0 || nDet < 0)
+    return fail(kErrArg, "namsa_multislice_noalloc: B=%d group=%d nDet=%d", B, group, nDet);
+  // scratch = [G planes of wave state][G x (Nx / narrowest tile) x nDet partial sums]
+  const size_t plane_bytes = static_cast<size_t>(Ny) * Nx * sizeof(float2);
+  const size_t det_bytes = static_cast<size_t>(Nx / kMinColTile) * nDet * sizeof(double);
+  const long long cap = static_cast<long long>(NAMSA_SCRATCH_BYTES / (plane_bytes + det_bytes));
+  if (cap < 1)
+    return fail(kErrArg, "namsa_multislice_noalloc: one %dx%d probe needs %zu bytes, the scratch holds %zu", Ny, Nx,
+                plane_bytes + det_bytes, static_cast<size_t>(NAMSA_SCRATCH_BYTES));
+  const long long want = (group > 0 && group < B) ? group : B;
+  if (group > 0 && want > cap)
+    return fail(kErrArg, "namsa_multislice_noalloc: group %lld exceeds the %lld probes of %dx%d (%d detectors) "
+                "the %zu-byte scratch holds", want, cap, Ny, Nx, nDet, static_cast<size_t>(NAMSA_SCRATCH_BYTES));
+  const int G = static_cast<int>(std::min(want, cap));
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  void* psi_ws = base;
+  double* det_ws = nDet > 0 ? reinterpret_cast<double*>(base + G * plane_bytes) : nullptr;
+  return multislice_impl(psi_k, pos_yx, B, Ny, Nx, kmax, t, S, pm, band_lo, band_hi, G, NAMSA_OUT_CBED, out_scale,
+                         psi_ws, cbed, det, nDet, det_ws, det_out, nullptr, nullptr, stream);
 }
 
 }  // extern "C"
