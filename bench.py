@@ -46,7 +46,8 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="run a 1 GiB device copy first (HBM counter calibration)")
     p.add_argument("--kernel-reps", type=int, default=40)
-    p.add_argument("--workload", choices=["config2", "config3", "config4", "config5", "potential3d", "records", "stream"],
+    p.add_argument("--workload", choices=["config1", "config2", "config3", "config4", "config5", "potential3d", "records",
+                                          "stream"],
                    default="config2",
                    help="config2: 4D-STEM at 512^2 (headline); config3: 48-material uneven batch; "
                         "config4: learned coarse-slice corrector vs plain at 256^2; config5: 2048^2 "
@@ -145,8 +146,57 @@ def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
             "slice_step_band_GBs": step_band, "row_ms": row_ms, "col_ms": col_ms}
 
 
+def host_cpu_info():
+    """The host the CPU baseline runs on: core counts, affinity, model, numpy."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"os_cpu_count": os.cpu_count(), "affinity": affinity, "model": model, "numpy": np.__version__}
+
+
+# Worker processes of the CPU baseline: the reference sizes its Pool as
+# min(os.cpu_count(), probes) (msa.py:295); one GPU's share of the box's host
+# is 16 cores, whatever os.cpu_count() reports there, so the pool is capped at that.
+CPU_SHARE = 16
+
+
+def cpu_config1():
+    """Config 1 in full on the CPU path: Si[111], 256^2, dz = 1.92 A, one probe
+    at the centre, 100 kV, 20 mrad (oracle.cpu_port: the numpy branch of
+    msa.py:280-361, plus the CBED readout of the exit wave), one process."""
+    from namsa_amd import MSAGPU, structures
+    from oracle import cpu_port
+    from oracle import msa_oracle as O
+
+    rec = structures.si111_supercell()
+    sim = MSAGPU(100.0, 0.020, rec, sampling=np.array([256, 256]), debye_waller=False)
+    kirk = np.load(ROOT / "namsa_amd" / "data" / "kirkland_params.npy")
+    sites, uniq = O.bin_sites(sim.supercell_xyz, sim.supercell_Z, 1.92, sim.pix_size, sim.dims)
+    tables = np.stack([O.atomic_table(kirk[z - 1], sim.pix_size) for z in uniq])
+    V = O.splat_slices(tables, sites, 256, 256).astype(np.float32)
+    t0 = time.perf_counter()
+    cb = cpu_port.multislice_cpu([(0.0, 0.0)], V, sim.sigma, sim.kmax, sim.Lambda, 1.92, sim.semi_ang,
+                                 bandwidth=1 / 3, probe_kw=dict(apert_smooth=80.0), processes=1, cbed=True)
+    dt = time.perf_counter() - t0
+    S = V.shape[0]
+    return {"value": S / dt, "unit": "probe-positions·slices/s", "wall_s": dt, "cores": 1, "kind": "port",
+            "finite": bool(np.isfinite(cb).all()),
+            "sample": f"config 1 in full: Si[111] {len(rec)} atoms, 1 probe x {S} slices at 256^2 + CBED readout"}
+
+
 def cpu_baseline(sim, n_probes):
-    """The reference CPU path (oracle.cpu_port) on n_probes raster points."""
+    """The reference CPU path (oracle.cpu_port, with the CBED readout) on
+    n_probes raster points of the 512^2 workload, plus config 1 in full."""
     from oracle import cpu_port
     from oracle import msa_oracle as O
 
@@ -154,24 +204,66 @@ def cpu_baseline(sim, n_probes):
     sites, uniq = O.bin_sites(sim.supercell_xyz, sim.supercell_Z, sim.slice_t, sim.pix_size, sim.dims)
     tables = np.stack([O.atomic_table(kirk[z - 1], sim.pix_size) for z in uniq])
     V = O.splat_slices(tables, sites, 512, 512).astype(np.float32)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    host = host_cpu_info()
+    cores = max(1, min(host["affinity"], CPU_SHARE))
     n = n_probes or 5 * cores                  # five points per core: about 10 s of CPU work
     idx = np.linspace(0, len(sim.probe_positions) - 1, n).astype(int)
     # the CPU path takes positions as (x, y) (msa.py:225)
     pos = sim.probe_positions[idx][:, ::-1]
     t0 = time.perf_counter()
-    cpu_port.multislice_cpu(pos, V, sim.sigma, sim.kmax, sim.Lambda, sim.slice_t, sim.semi_ang,
-                            bandwidth=1 / 3, probe_kw=dict(apert_smooth=80.0), processes=min(cores, n))
+    cb = cpu_port.multislice_cpu(pos, V, sim.sigma, sim.kmax, sim.Lambda, sim.slice_t, sim.semi_ang,
+                                 bandwidth=1 / 3, probe_kw=dict(apert_smooth=80.0), processes=min(cores, n),
+                                 cbed=True)
     dt = time.perf_counter() - t0
     S = V.shape[0]
     return {"value": n * S / dt, "unit": "probe-positions·slices/s", "cores": min(cores, n),
-            "kind": "port",
-            "sample": f"{n} raster points x {S} slices at 512^2 (oracle/cpu_port.py, numpy "
-                      f"{np.__version__}, fork Pool of {min(cores, n)}), {dt:.1f} s"}
+            "kind": "port", "host": host, "finite": bool(np.isfinite(cb).all()),
+            "sample": f"{n} raster points x {S} slices at 512^2 with the CBED readout (oracle/cpu_port.py, "
+                      f"numpy {np.__version__}, fork Pool of {min(cores, n)}: the reference's "
+                      f"min(cpu_count, probes), capped at the {CPU_SHARE}-core share of one GPU), {dt:.1f} s",
+            "config1": cpu_config1()}
+
+
+def bench_config1(args):
+    """Config 1: Si[111] 38 x 39 x 200 A (15,360 atoms), 256^2, dz = 1.92 A
+    (104 slices), 100 kV, 20 mrad, one probe at the centre -- the reference's
+    CPU-path workload -- through the fused GPU driver, beside the CPU path
+    timed in full."""
+    import torch
+
+    from namsa_amd import DEFAULT_PROBE, MSAGPU, structures
+
+    sim = MSAGPU(100.0, 0.020, structures.si111_supercell(), sampling=np.array([256, 256]), debye_waller=False)
+    sim.setup_device(0)
+    sim.calc_atomic_potentials()
+    sim.build_potential_slices(1.92)
+    sim.build_probe(dict(DEFAULT_PROBE))
+    sim.probe_positions = np.zeros((1, 2))
+    sim.num_probes = np.int32(1)
+    sim.grid_steps = np.array([1, 1])
+    sim.plan_simulation()
+    S = int(sim.num_slices)
+    for _ in range(max(1, args.warmup)):
+        sim.multislice(bandwidth=1 / 3)
+    reps = max(5, args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.multislice(bandwidth=1 / 3)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    total = float(sim.probes.sum())
+    cpu = None if args.no_cpu_baseline else cpu_config1()
+    out = {
+        "metric": "probe-positions·slices/s, config 1 (single probe, CPU-path workload)",
+        "value": S / wall, "unit": "probe-positions·slices/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (Si[111] supercell, deterministic)",
+        "config": {"workload": "config 1: Si[111] 256^2, 104 slices, 1 probe, CBED (drop-in multislice call, "
+                               "host overheads included)", "grid": [256, 256], "slices": S, "probes": 1},
+        "cbed_sum": total, "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_potential3d(args):
@@ -662,6 +754,8 @@ def main():
 
     import namsa_amd._lib as L
 
+    if args.workload == "config1":
+        return bench_config1(args)
     if args.workload == "config3":
         return bench_config3(args)
     if args.workload == "config4":

@@ -72,8 +72,15 @@ def build_probe_cpu(kmax, Ny, Nx, lam, semi_angle, pos, smooth_apert=True,
     return psi.astype(np.complex64)
 
 
-def propagate_beam(pos, V, sigma, kmax, lam, dz, semi_angle, bandwidth, probe_kw):
-    """One probe through all slices (msa.py:320-361, numpy branch)."""
+def cbed_readout(psi):
+    """The CBED the reference's notebooks take of a CPU exit wave:
+    |fftshift(fft2(psi))|^2, unnormalised, float32."""
+    return (np.abs(np.fft.fftshift(np.fft.fft2(psi))) ** 2).astype(np.float32)
+
+
+def propagate_beam(pos, V, sigma, kmax, lam, dz, semi_angle, bandwidth, probe_kw, cbed=False):
+    """One probe through all slices (msa.py:320-361, numpy branch); with
+    cbed=True also the readout of the exit wave (cbed_readout)."""
     Ny, Nx = V.shape[1:]
     propag = build_propagator(kmax, Ny, Nx, lam, dz)
     blim = bandwidth_limit_mask((Ny, Nx), bandwidth)
@@ -81,7 +88,7 @@ def propagate_beam(pos, V, sigma, kmax, lam, dz, semi_angle, bandwidth, probe_kw
     slices = np.exp(1j * sigma * V).astype(np.complex64)
     for trans in slices[::-1]:
         psi = np.fft.ifft2(np.fft.fft2(trans * psi) * blim * propag)
-    return psi
+    return cbed_readout(psi) if cbed else psi
 
 
 def _worker(args):
@@ -89,17 +96,18 @@ def _worker(args):
 
 
 def multislice_cpu(positions, V, sigma, kmax, lam, dz, semi_angle,
-                   bandwidth=1 / 3, probe_kw=None, processes=None):
-    """Exit waves for all positions with a fork Pool (msa.py:289-312)."""
+                   bandwidth=1 / 3, probe_kw=None, processes=None, cbed=False):
+    """Exit waves (or, with cbed=True, CBEDs) for all positions with a fork
+    Pool (msa.py:289-312)."""
     probe_kw = probe_kw or {}
     positions = [tuple(p) for p in np.asarray(positions, dtype=np.float64)]
     if processes is None:
         processes = min(os.cpu_count() or 1, len(positions))
     if processes <= 1:
         return np.array([propagate_beam(p, V, sigma, kmax, lam, dz, semi_angle,
-                                        bandwidth, probe_kw) for p in positions])
+                                        bandwidth, probe_kw, cbed) for p in positions])
     _SHARED["V"] = np.asarray(V, dtype=np.float32)
-    _SHARED["rest"] = (sigma, kmax, lam, dz, semi_angle, bandwidth, probe_kw)
+    _SHARED["rest"] = (sigma, kmax, lam, dz, semi_angle, bandwidth, probe_kw, cbed)
     chunk = max(1, len(positions) // processes)
     ctx = mp.get_context("fork")
     with ctx.Pool(processes=processes) as pool:

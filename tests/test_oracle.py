@@ -389,6 +389,21 @@ def _grid3(nz, ny, nx, dz=0.25, dy=0.1, dx=0.12):
     return np.arange(nz) * dz, np.arange(ny) * dy, np.arange(nx) * dx
 
 
+def test_cpu_cbed_readout_parseval_and_delta():
+    # |fftshift(fft2(psi))|^2: Parseval gives N^2 sum |psi|^2; a delta at the origin is flat;
+    # a plane wave of one cycle across the grid lands one pixel right of the shifted centre
+    rng = np.random.default_rng(3)
+    psi = (rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))).astype(np.complex64)
+    cb = cpu_port.cbed_readout(psi)
+    assert cb.dtype == np.float32 and cb.sum() == pytest.approx(256 * np.sum(np.abs(psi) ** 2), rel=1e-5)
+    d = np.zeros((16, 16), np.complex64)
+    d[0, 0] = 1
+    np.testing.assert_array_equal(cpu_port.cbed_readout(d), np.ones((16, 16), np.float32))
+    x = np.exp(2j * np.pi * np.arange(16) / 16)[None, :] * np.ones((16, 1))
+    cb = cpu_port.cbed_readout(x.astype(np.complex64))
+    assert np.unravel_index(np.argmax(cb), cb.shape) == (8, 9)
+
+
 def test_integrate_slices_closed_forms():
     # constant c: each slice integrates c over its own (d - 1) intervals
     z, y, x = _grid3(17, 6, 5)      # extent 4.0, dz = 1.0 -> S = 4, d = 4, 16 of 17 samples used
