@@ -342,6 +342,41 @@ int namsa_modes_expand(const float* Ys, float* E, int C, int B, int m, void* str
 int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B, int hw, float s,
                           void* stream);
 
+/*
+ * Spectral (one-shot) corrector: works on the centred exit spectrum Psi(k) of
+ * the coarse path, |Psi|^2 being the CBED, inside the window
+ * [r0, r0+H) x [c0, c0+W) of the Ny x Nx plane that bounds the bandwidth mask.
+ * psi_c, psi_0, target, cbed: full planes [B][Ny][Nx]; u, out, gout:
+ * channel-major [ch][B*H*W] over the window.
+ *
+ * namsa_spectral_pack: network inputs
+ *   u[0..1] = s Psi_c (coarse exit spectrum), u[2..3] = s Psi_0 (the probe's
+ *   exit spectrum through vacuum of the same thickness), u[4] = phase_scale *
+ *   |k|^2 on sample b's linspace k-grid of extent kmax[b] (device [B]),
+ *   u[5] = thick[b] * thick_scale (device [B]), then with
+ *   opts & 1: log1p(vscale |vhat[vidx[b]]|) (vhat: [nv][Ny][Nx] complex,
+ *   centred spectra of the projected potential; vidx NULL = plane 0; an
+ *   index outside [0, nv) gives NaN, never a read outside vhat), with
+ *   opts & 2: |s Psi_c|.
+ * namsa_spectral_loss: with Psi = Psi_c (1 + o), o = out[0] + i out[1],
+ *   I = |s Psi|^2 and F = s^2 target: sums[2b] = sum (I - F)^2,
+ *   sums[2b+1] = sum F^2 over the window; gout (optional) = gradient of
+ *   mean_b sums[2b] / sums[2b+1] with respect to out.
+ * namsa_spectral_apply: cbed = |Psi_c (1 + o)|^2 * scale in the window,
+ *   |Psi_c|^2 * scale elsewhere.
+ * namsa_real_to_complex: out = (scale v, 0), times (-1)^(y+x) if checker (so
+ *   that its forward transform comes out centred), over [B][Ny][Nx].
+ */
+int namsa_spectral_pack(const void* psi_c, const void* psi_0, const void* vhat, const int* vidx, int nv, int B, int Ny,
+                        int Nx, int r0, int c0, int H, int W, const float* kmax, float s, float phase_scale,
+                        const float* thick, float thick_scale, float vscale, int opts, float* u, void* stream);
+int namsa_spectral_loss(const float* out, const void* psi_c, const float* target, int B, int Ny, int Nx, int r0,
+                        int c0, int H, int W, float s, double* sums, float* gout, void* stream);
+int namsa_spectral_apply(const float* out, const void* psi_c, int B, int Ny, int Nx, int r0, int c0, int H, int W,
+                         float scale, float* cbed, void* stream);
+int namsa_real_to_complex(const float* v, int B, int Ny, int Nx, float scale, int checker, void* out,
+                          void* stream);
+
 /* ---- training records (the 4D-STEM dataset step after the slice loop) ---- */
 
 /*

@@ -74,6 +74,11 @@ SIGNATURES = {
     "namsa_potential_record_shape": (_i, [_i, _i, _vp, _vp]),
     "namsa_potential_record_workspace_bytes": (_ll, [_i, _i]),
     "namsa_potential_record": (_i, [_vp, _i, _i, _i, _d, _d, _vp, _vp, _vp]),
+    "namsa_spectral_pack": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _f, _f, _vp, _f, _f, _i,
+                                 _vp, _vp]),
+    "namsa_spectral_loss": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp]),
+    "namsa_spectral_apply": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp]),
+    "namsa_real_to_complex": (_i, [_vp, _i, _i, _i, _f, _i, _vp, _vp]),
     "namsa_modes_combine": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "namsa_modes_expand": (_i, [_vp, _vp, _i, _i, _i, _vp]),
 }

@@ -542,3 +542,382 @@ def evaluate(net, pool, batch=8):
         num += s[:, 0].sum()
         den += s[:, 1].sum()
     return float(num / den), np.asarray(per)
+
+
+# ====================================================================== spectral corrector
+# One correction per probe, applied to the coarse path's exit spectrum (the
+# complex amplitude whose squared modulus is the CBED):
+#
+#     Psi = Psi_c (1 + o),   o = out[0] + i out[1] = FNO(u)
+#
+# over the window that bounds the bandwidth mask.  Inputs u (per k-pixel):
+# s Psi_c, the probe's exit spectrum s Psi_0 through vacuum of the same
+# thickness (the reference the network compares against), the phase
+# pi lambda dz_c |k|^2 of one coarse propagator step, the thickness, and the
+# magnitude of the projected potential's spectrum (where the specimen
+# scatters).  s = sqrt(Ny Nx) puts |s Psi|^2 at mean 1 per pixel (the CBED
+# sums to 1).  Training minimises the mean over samples of the CBED's
+# relative squared error, sum (I - F)^2 / sum F^2, against the fine path.
+
+SPECTRAL_PHASE_DIV = 10.0      # u[4] = pi lambda dz_c |k|^2 / 10
+SPECTRAL_THICK_SCALE = 1.0 / 400.0
+SPECTRAL_VSCALE = 100.0        # u[6] = log1p(100 |FFT(sigma V_T)| / (Ny Nx))
+OPT_POTENTIAL, OPT_AMPLITUDE = 1, 2
+
+
+def band_window(mask):
+    """(r0, c0, H, W): bounding box of the nonzero bandwidth mask [Ny][Nx] (host array)."""
+    rows = np.nonzero(np.asarray(mask).any(axis=1))[0]
+    cols = np.nonzero(np.asarray(mask).any(axis=0))[0]
+    return int(rows[0]), int(cols[0]), int(rows[-1] - rows[0] + 1), int(cols[-1] - cols[0] + 1)
+
+
+class SpectralMaterial:
+    """One structure prepared for fine (dz) and coarse (f * dz, f in factors)
+    stepping and for the spectral corrector's inputs.
+
+    Holds the fine slices t_f, coarse slices t_c[f] (the same site table
+    binned f * dz thick), the propagators, the bandwidth window and the fine
+    site table (for projected potentials of any thickness).
+    """
+
+    def __init__(self, sim, dz=1.0, factors=(8,), bandwidth=1 / 3, probe_dict=None):
+        torch = _torch()
+        self.sim = sim
+        self.dz, self.bandwidth = float(dz), float(bandwidth)
+        self.factors = tuple(int(f) for f in factors)
+        sim._check_grid()
+        if not hasattr(sim, "tables_device"):
+            sim.calc_atomic_potentials()
+        Ny, Nx = sim._grid
+        self.Ny, self.Nx = Ny, Nx
+        dev = sim.device
+        sim.build_potential_slices(dz)
+        self.t_f = sim.potential_slices_device
+        self.S_f = int(sim.num_slices)
+        sites, offs = sim.site_table(dz)
+        self.sites_d = torch.from_numpy(sites.reshape(-1)).to(dev) if len(sites) else None
+        self.offsets = offs
+        self.pm_f = sim.build_propagator(bandwidth).clone()
+        self.t_c, self.pm_c, self.S_c = {}, {}, {}
+        for f in self.factors:
+            sim.build_potential_slices(dz * f)
+            self.t_c[f] = sim.potential_slices_device
+            self.S_c[f] = int(sim.num_slices)
+            if self.S_c[f] * f > self.S_f:
+                raise NamsaError("coarse slices extend past the fine ones")
+            self.pm_c[f] = sim.build_propagator(bandwidth).clone()
+        self.mask = sim.mask_device.clone()
+        self.band = sim.band
+        self.window = band_window(self.mask.cpu().numpy())
+        if not hasattr(sim, "psi_k_device") or probe_dict is not None:
+            sim.build_probe(probe_dict) if probe_dict is not None else sim.build_probe()
+        self.norm = float(sim.normalization)
+        self.lam = float(sim.Lambda)
+        self.ones = torch.ones((1, Ny, Nx), dtype=torch.complex64, device=dev)
+
+    def _slices(self, factor):
+        return (self.t_f, self.pm_f) if factor in (None, 1) else (self.t_c[factor], self.pm_c[factor])
+
+    def exit_spectra(self, pos, S, factor, out):
+        """Centred exit spectra [B][Ny][Nx] after S slices of dz * factor
+        (factor None or 1: the fine slices): exit waves scaled
+        1/sqrt(Ny Nx norm), then the forward transform, so that |out|^2 is
+        the CBED."""
+        B = pos.shape[0]
+        t, pm = self._slices(factor)
+        scale = np.float32(1.0 / math.sqrt(self.Ny * self.Nx * self.norm))
+        call("namsa_multislice", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
+             np.float32(self.sim.kmax), ptr(t), int(S), ptr(pm), self.band[0], self.band[1], B, _lib.OUT_EXIT,
+             scale, ptr(out), None, None, 0, None, None, stream_handle())
+        call("namsa_fft2d", ptr(out), B, self.Ny, self.Nx, 0, stream_handle())
+        return out
+
+    def cbed(self, pos, S, factor, out):
+        """Plain-path CBEDs [B][Ny][Nx] after S slices of dz * factor (the fused driver)."""
+        torch = _torch()
+        B = pos.shape[0]
+        t, pm = self._slices(factor)
+        ws = torch.empty((B, self.Ny, self.Nx), dtype=torch.complex64, device=out.device)
+        call("namsa_multislice", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
+             np.float32(self.sim.kmax), ptr(t), int(S), ptr(pm), self.band[0], self.band[1], B, _lib.OUT_CBED,
+             np.float32(1.0 / self.norm), ptr(ws), ptr(out), None, 0, None, None, stream_handle())
+        return out
+
+    def vacuum_spectra(self, pos, thickness, out):
+        """Exit spectra of the same probes through vacuum of `thickness`: one
+        step with t = 1 and the propagator over the whole thickness."""
+        torch = _torch()
+        pm_t = torch.empty((self.Ny, self.Nx), dtype=torch.complex64, device=out.device)
+        call("namsa_propagator", self.Ny, self.Nx, np.float32(self.sim.kmax),
+             np.float32(self.bandwidth * self.sim.kmax), np.float32(self.lam), np.float32(thickness), ptr(pm_t),
+             None, stream_handle())
+        B = pos.shape[0]
+        scale = np.float32(1.0 / math.sqrt(self.Ny * self.Nx * self.norm))
+        call("namsa_multislice", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
+             np.float32(self.sim.kmax), ptr(self.ones), 1, ptr(pm_t), self.band[0], self.band[1], B,
+             _lib.OUT_EXIT, scale, ptr(out), None, None, 0, None, None, stream_handle())
+        call("namsa_fft2d", ptr(out), B, self.Ny, self.Nx, 0, stream_handle())
+        return out
+
+    def potential_spectrum(self, S, out):
+        """Centred spectrum of sigma V over the first S fine slices, [Ny][Nx] complex."""
+        torch = _torch()
+        sim = self.sim
+        Py, Px = sim.table_shape
+        v = torch.empty((1, self.Ny, self.Nx), dtype=torch.float32, device=out.device)
+        offs = torch.tensor([0, int(self.offsets[int(S)])], dtype=torch.int32, device=out.device)
+        call("namsa_build_potential", ptr(sim.tables_device), len(sim.unique_Z), Py, Px, ptr(self.sites_d),
+             ptr(offs), 1, self.Ny, self.Nx, ptr(v), stream_handle())
+        call("namsa_real_to_complex", ptr(v), 1, self.Ny, self.Nx, np.float32(sim.sigma), 1, ptr(out),
+             stream_handle())
+        call("namsa_fft2d", ptr(out), 1, self.Ny, self.Nx, 0, stream_handle())
+        return out
+
+
+class SpectralCorrector:
+    """The spectral corrector network (an FnoNet over the band window) and its
+    input packing, loss and application."""
+
+    def __init__(self, Ny, Nx, window, kmax, lam, dz_coarse, opts=OPT_POTENTIAL, C=32, L=4, m=16, Hq=64,
+                 seed=0, device="cuda", factor=8):
+        self.Ny, self.Nx = int(Ny), int(Nx)
+        self.factor = int(factor)
+        self.window = tuple(int(v) for v in window)
+        self.kmax, self.lam, self.dz_coarse, self.opts = float(kmax), float(lam), float(dz_coarse), int(opts)
+        cin = 6 + (1 if opts & OPT_POTENTIAL else 0) + (1 if opts & OPT_AMPLITUDE else 0)
+        _, _, H, W = self.window
+        self.net = FnoNet(H, W, cin=cin, C=C, L=L, m=m, Hq=Hq, seed=seed, device=device, derivs=False)
+        self.s = float(math.sqrt(self.Ny * self.Nx))
+        self.phase_scale = float(math.pi * self.lam * self.dz_coarse / SPECTRAL_PHASE_DIV)
+
+    def config(self):
+        n = self.net
+        return dict(Ny=self.Ny, Nx=self.Nx, window=list(self.window), kmax=self.kmax, lam=self.lam,
+                    dz_coarse=self.dz_coarse, factor=self.factor, opts=self.opts, C=n.C, L=n.L, m=n.m, Hq=n.Hq)
+
+    def save(self, path):
+        """Weights as bfloat16 bit patterns (the network's operand precision) plus the configuration."""
+        import json
+        theta = self.net.theta.detach().cpu().numpy().astype(np.float32)
+        bits = theta.view(np.uint32)
+        rounded = ((bits + 0x7FFF + ((bits >> 16) & 1)) >> 16).astype(np.uint16)   # round to nearest even
+        np.savez_compressed(path, theta_bf16=rounded, config=np.array(json.dumps(self.config())))
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        import json
+        torch = _torch()
+        d = np.load(path, allow_pickle=False)
+        cfg = json.loads(str(d["config"]))
+        obj = cls(cfg["Ny"], cfg["Nx"], cfg["window"], cfg["kmax"], cfg["lam"], cfg["dz_coarse"], cfg["opts"],
+                  C=cfg["C"], L=cfg["L"], m=cfg["m"], Hq=cfg["Hq"], device=device, factor=cfg["factor"])
+        theta = (d["theta_bf16"].astype(np.uint32) << 16).view(np.float32)
+        if theta.size != obj.net.n_params:
+            raise NamsaError(f"{path}: {theta.size} weights, the network has {obj.net.n_params}")
+        obj.net.theta.copy_(torch.from_numpy(theta))
+        return obj
+
+    def pack(self, psi_c, psi_0, vhat, vidx, kmax, thick, B):
+        r0, c0, H, W = self.window
+        self.net.ensure(B)
+        nv = int(vhat.shape[0]) if vhat is not None and vhat.dim() == 3 else 1
+        call("namsa_spectral_pack", ptr(psi_c), ptr(psi_0), ptr(vhat), ptr(vidx), nv, B, self.Ny, self.Nx, r0, c0, H,
+             W, ptr(kmax), np.float32(self.s), np.float32(self.phase_scale), ptr(thick),
+             np.float32(SPECTRAL_THICK_SCALE), np.float32(SPECTRAL_VSCALE / (self.Ny * self.Nx)), self.opts,
+             ptr(self.net.u), self.net.st)
+
+    def loss(self, psi_c, target, B, grad=False):
+        r0, c0, H, W = self.window
+        call("namsa_spectral_loss", ptr(self.net.out), ptr(psi_c), ptr(target), B, self.Ny, self.Nx, r0, c0, H, W,
+             np.float32(self.s), ptr(self.net.sums), ptr(self.net.gout) if grad else None, self.net.st)
+        return self.net.sums[:2 * B].view(B, 2)
+
+    def apply(self, psi_c, B, cbed):
+        r0, c0, H, W = self.window
+        call("namsa_spectral_apply", ptr(self.net.out), ptr(psi_c), B, self.Ny, self.Nx, r0, c0, H, W,
+             np.float32(1.0), ptr(cbed), self.net.st)
+        return cbed
+
+    def correct(self, psi_c, psi_0, vhat, vidx, kmax, thick, cbed):
+        """CBEDs [B][Ny][Nx] of corrected coarse exit spectra (kmax, thick: device [B] float32)."""
+        B = psi_c.shape[0]
+        self.pack(psi_c, psi_0, vhat, vidx, kmax, thick, B)
+        self.net.forward(B)
+        return self.apply(psi_c, B, cbed)
+
+
+class SpectralPairPool:
+    """Training pairs resident in HBM: per coarsening factor the coarse exit
+    spectra, and shared by every factor the vacuum spectra, the fine CBEDs
+    (targets), the thicknesses and an index into the potential spectra."""
+
+    def __init__(self, factors=(8,)):
+        self.factors = tuple(int(f) for f in factors)
+        self.pc = {f: [] for f in self.factors}
+        self.p0, self.f, self.th, self.vi, self.vh, self.km = [], [], [], [], [], []
+
+    def add(self, mat, pos, thicknesses):
+        """Pairs of the probes at pos at each thickness (a multiple of every
+        factor's coarse slice)."""
+        torch = _torch()
+        dev = pos.device
+        B = pos.shape[0]
+        for T in thicknesses:
+            S_f = int(round(T / mat.dz))
+            p0 = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.complex64, device=dev)
+            f = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.float32, device=dev)
+            vh = torch.empty((mat.Ny, mat.Nx), dtype=torch.complex64, device=dev)
+            mat.vacuum_spectra(pos, S_f * mat.dz, p0)
+            mat.cbed(pos, S_f, None, f)
+            mat.potential_spectrum(S_f, vh)
+            for fac in self.factors:
+                if S_f % fac:
+                    raise NamsaError(f"thickness {T} is not a whole number of {fac * mat.dz} A slices")
+                pc = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.complex64, device=dev)
+                mat.exit_spectra(pos, S_f // fac, fac, pc)
+                self.pc[fac].extend(pc[b] for b in range(B))
+            vi = len(self.vh)
+            self.vh.append(vh)
+            for b in range(B):
+                self.p0.append(p0[b])
+                self.f.append(f[b])
+                self.th.append(float(S_f * mat.dz))
+                self.vi.append(vi)
+                self.km.append(float(mat.sim.kmax))
+
+    def finalize(self):
+        torch = _torch()
+        self.PC = {fac: torch.stack(v) for fac, v in self.pc.items()}
+        self.P0 = torch.stack(self.p0)
+        self.F = torch.stack(self.f)
+        self.VH = torch.stack(self.vh)
+        dev = self.P0.device
+        self.TH = torch.tensor(self.th, dtype=torch.float32, device=dev)
+        self.VI = torch.tensor(self.vi, dtype=torch.int32, device=dev)
+        self.KM = torch.tensor(self.km, dtype=torch.float32, device=dev)
+        self.pc = self.p0 = self.f = self.vh = None
+        return self
+
+    def __len__(self):
+        return len(self.th)
+
+    def batch(self, idx, factor):
+        torch = _torch()
+        ii = torch.as_tensor(np.asarray(idx), device=self.P0.device)
+        return (self.PC[factor].index_select(0, ii), self.P0.index_select(0, ii), self.F.index_select(0, ii),
+                self.TH.index_select(0, ii), self.VI.index_select(0, ii), self.KM.index_select(0, ii))
+
+
+def spectral_batch_loss(corr, pool, idx, grad=False):
+    """Per-sample (sum (I - F)^2, sum F^2) of pool samples idx under the
+    network (and with grad the gradient of the mean relative error in net.grad)."""
+    B = len(idx)
+    pc, p0, f, th, vi, km = pool.batch(idx, corr.factor)
+    corr.pack(pc, p0, pool.VH, vi, km, th, B)
+    corr.net.forward(B)
+    sums = corr.loss(pc, f, B, grad=grad)
+    if grad:
+        corr.net.backward(B)
+    return sums
+
+
+def spectral_train(corr, pool, steps, batch=16, lr=2e-3, seed=0, log_every=0, log=print):
+    """Adam with cosine decay on random minibatches; returns (step, mean rel-L2) log points."""
+    rng = np.random.default_rng(seed)
+    corr.net.ensure(batch)
+    hist = []
+    for it in range(steps):
+        idx = rng.choice(len(pool), size=batch, replace=False)
+        sums = spectral_batch_loss(corr, pool, idx, grad=True)
+        corr.net.adam(lr * 0.5 * (1 + math.cos(math.pi * it / steps)))
+        if log_every and (it % log_every == 0 or it == steps - 1):
+            r = float((sums[:, 0] / sums[:, 1]).sqrt().mean())
+            hist.append((it, r))
+            log(f"step {it:5d}  CBED rel-L2 {r:.4f}")
+    return hist
+
+
+def spectral_evaluate(corr, pool, batch=16):
+    """Per-sample CBED rel-L2 over the window, learned and uncorrected (o = 0)."""
+    torch = _torch()
+    corr.net.ensure(batch)
+    learned, coarse = [], []
+    for i0 in range(0, len(pool), batch):
+        idx = list(range(i0, min(len(pool), i0 + batch)))
+        B = len(idx)
+        s = spectral_batch_loss(corr, pool, idx).cpu().numpy()
+        learned.extend(np.sqrt(s[:, 0] / s[:, 1]))
+        pc, _, f, _, _, _ = pool.batch(idx, corr.factor)
+        call("namsa_fill", ptr(corr.net.out), 2 * B * corr.window[2] * corr.window[3], np.float32(0.0), corr.net.st)
+        s0 = corr.loss(pc, f, B).cpu().numpy()
+        coarse.extend(np.sqrt(s0[:, 0] / s0[:, 1]))
+    return np.asarray(learned), np.asarray(coarse)
+
+
+# ---------------------------------------------------------------------- training data
+def random_crystal_sim(seed, grid=256, pix=0.125, kv=200.0, semi=0.025, thickness=(40.0, 400.0), device=None):
+    """A seeded random crystal (cubic / tetragonal / hexagonal lattice, 1-8
+    basis atoms, Z in [3, 83]) grid * pix wide and uniform in `thickness`
+    thick, prepared as a drop-in simulation at grid^2 (the structures of the
+    corrector's training pairs).  Returns (sim, rng) with the rng advanced
+    past the structure, for the pair's remaining random choices."""
+    from . import structures
+    from .msa import MSAGPU
+
+    rng = np.random.default_rng(int(seed))
+    T = float(rng.uniform(*thickness))
+    rec, _ = structures.random_crystal(rng, grid * pix, T + 0.5)
+    sim = MSAGPU(kv, semi, rec, sampling=np.array([grid, grid]), debye_waller=False)
+    sim.setup_device(0 if device is None else device)
+    sim.calc_atomic_potentials()
+    return sim, rng, T
+
+
+def polycrystal_sim(seed, grid=2048, kv=300.0, semi=0.030, lateral=128.0, thickness=1000.0, device=None):
+    """A seeded 4-grain Si / GaAs / Au polycrystal lateral x lateral x
+    thickness A (the large-scale comparison's structure type) as a drop-in
+    simulation at grid^2.  Returns (sim, rng, thickness) like random_crystal_sim."""
+    from . import structures
+    from .msa import MSAGPU
+
+    rec, _ = structures.polycrystal(seed=int(seed), lateral=lateral, thickness=thickness)
+    sim = MSAGPU(kv, semi, rec, sampling=np.array([grid, grid]), debye_waller=False)
+    sim.setup_device(0 if device is None else device)
+    sim.calc_atomic_potentials()
+    return sim, np.random.default_rng(int(seed) + 7919), float(thickness)
+
+
+def make_spectral_pools(seeds, factors=(8,), probes=4, grid=256, pix=0.125, kv=200.0, semi=0.025,
+                        thickness=(40.0, 400.0), dz=1.0, log=None, builder=None):
+    """One pair pool over every coarsening factor, from seeded structures.
+
+    builder(seed) -> (sim, rng, T) makes a structure (default: a random
+    crystal, random_crystal_sim with the other arguments).  Per structure:
+    `probes` random positions within the central half of the cell and two
+    thickness snapshots -- the whole structure, and a random thickness from
+    thickness[0] up, both in whole multiples of 8 dz -- so 2 * probes pairs.
+    The fine targets, vacuum spectra and potential spectra are shared by
+    every factor; only the coarse exit spectra differ.
+    """
+    torch = _torch()
+    if builder is None:
+        builder = lambda sd: random_crystal_sim(sd, grid, pix, kv, semi, thickness)   # noqa: E731
+    pool = SpectralPairPool(factors)
+    unit = 8 * dz
+    mats = None
+    for n, seed in enumerate(seeds):
+        mats = None
+        sim, rng, T = builder(seed)
+        L = float(min(sim.dims[:2]))
+        pos = torch.from_numpy(rng.uniform(-L / 4, L / 4, (probes, 2)).astype(np.float32)).to(sim.device)
+        mats = SpectralMaterial(sim, dz=dz, factors=factors)
+        units = int(np.floor(mats.S_f * dz / unit))
+        lo = min(units, int(np.ceil(thickness[0] / unit)))
+        snaps = sorted({units, int(rng.integers(lo, units + 1))})
+        pool.add(mats, pos, [u * unit for u in snaps])
+        if log and (n % 100 == 0 or n == len(seeds) - 1 or len(seeds) <= 20):
+            log(f"  structure {n + 1}/{len(seeds)} (seed {seed}): {len(sim.supercell_Z)} atoms, "
+                f"snapshots {[u * unit for u in snaps]} A, {len(pool)} pairs")
+        del sim
+    return pool.finalize(), mats

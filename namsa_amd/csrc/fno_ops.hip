@@ -176,8 +176,164 @@ __global__ void modes_expand_kernel(const float* __restrict__ Ys, float* __restr
   }
 }
 
+// ------------------------------------------------------------------ spectral corrector
+// The spectral corrector works on the centred exit spectrum Psi(k) of the
+// coarse path (|Psi|^2 is the CBED), inside the window [r0, r0+H) x [c0, c0+W)
+// that bounds the bandwidth mask.  Planes are full [B][Ny][Nx]; the network's
+// arrays are channel-major [ch][B*H*W] over the window.
+struct Window {
+  int Ny, Nx, r0, c0, H, W;
+};
+
+__device__ __forceinline__ long long plane_index(const Window& w, long long b, int y, int x) {
+  return (b * w.Ny + (w.r0 + y)) * static_cast<long long>(w.Nx) + (w.c0 + x);
+}
+
+// u[0..1] = s Psi_c, u[2..3] = s Psi_0, u[4] = phase_scale * k^2 (k on sample b's linspace grid of
+// extent kmax[b]), u[5] = thick[b] * thick_scale,
+// then (opts & 1) log1p(vscale |vhat[vidx[b]]|) (NaN for an index outside [0, nv)), (opts & 2) |s Psi_c|
+__global__ void spectral_pack_kernel(const float2* __restrict__ pc, const float2* __restrict__ p0,
+                                     const float2* __restrict__ vhat, const int* __restrict__ vidx, int nv, Window w,
+                                     int B, const float* __restrict__ kmax, float s, float phase_scale,
+                                     const float* __restrict__ thick, float thick_scale, float vscale, int opts,
+                                     float* __restrict__ u) {
+  const long long hw = static_cast<long long>(w.H) * w.W;
+  const long long n = hw * B;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+    const long long b = i / hw;
+    const int y = static_cast<int>((i % hw) / w.W), x = static_cast<int>(i % w.W);
+    const long long j = plane_index(w, b, y, x);
+    const float2 c = pc[j], z = p0[j];
+    u[i] = c.x * s;
+    u[n + i] = c.y * s;
+    u[2 * n + i] = z.x * s;
+    u[3 * n + i] = z.y * s;
+    const double km = kmax[b];
+    const double ky = k_lin(w.r0 + y, km, w.Ny), kx = k_lin(w.c0 + x, km, w.Nx);
+    u[4 * n + i] = static_cast<float>(phase_scale * (ky * ky + kx * kx));
+    u[5 * n + i] = thick[b] * thick_scale;
+    int ch = 6;
+    if (opts & 1) {
+      const long long pv = vidx ? vidx[b] : 0;
+      if (pv >= 0 && pv < nv) {
+        const float2 v = vhat[(pv * w.Ny + (w.r0 + y)) * static_cast<long long>(w.Nx) + (w.c0 + x)];
+        u[ch * n + i] = log1pf(vscale * sqrtf(v.x * v.x + v.y * v.y));
+      } else {
+        u[ch * n + i] = __builtin_nanf("");
+      }
+      ++ch;
+    }
+    if (opts & 2) u[ch * n + i] = sqrtf(c.x * c.x + c.y * c.y) * s;
+  }
+}
+
+__global__ void zero_doubles_kernel(double* __restrict__ x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// I = |s Psi_c (1 + o)|^2, F = s^2 target over the window; per sample
+// sums[2b] += sum (I - F)^2 and sums[2b+1] += sum F^2 (sums zeroed first).
+__global__ void __launch_bounds__(256)
+spectral_sums_kernel(const float* __restrict__ out, const float2* __restrict__ pc, const float* __restrict__ tgt,
+                     Window w, float s, double* __restrict__ sums) {
+  __shared__ double red[2][4];
+  const long long hw = static_cast<long long>(w.H) * w.W;
+  const long long n = hw * gridDim.y;
+  const long long b = blockIdx.y;
+  const float s2 = s * s;
+  double num = 0.0, den = 0.0;
+  for (long long p = blockIdx.x * 256LL + threadIdx.x; p < hw; p += 256LL * gridDim.x) {
+    const long long i = b * hw + p;
+    const int y = static_cast<int>(p / w.W), x = static_cast<int>(p % w.W);
+    const long long j = plane_index(w, b, y, x);
+    const float2 c = pc[j];
+    const float a = 1.0f + out[i], q = out[n + i];
+    const float re = c.x * a - c.y * q, im = c.x * q + c.y * a;
+    const float I = (re * re + im * im) * s2, F = tgt[j] * s2;
+    num += double(I - F) * double(I - F);
+    den += double(F) * double(F);
+  }
+  num = wave_sum(num);
+  den = wave_sum(den);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = num;
+    red[1][threadIdx.x >> 6] = den;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[2 * b], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+    atomicAdd(&sums[2 * b + 1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+  }
+}
+
+// gout = d/d(o_re, o_im) of mean_b sum (I - F)^2 / sum F^2:
+// dI/do_re = 2 s^2 Re(conj(Psi) Psi_c), dI/do_im = -2 s^2 Im(conj(Psi) Psi_c), Psi = Psi_c (1 + o)
+__global__ void spectral_grad_kernel(const float* __restrict__ out, const float2* __restrict__ pc,
+                                     const float* __restrict__ tgt, Window w, int B, float s,
+                                     const double* __restrict__ sums, float* __restrict__ gout) {
+  const long long hw = static_cast<long long>(w.H) * w.W;
+  const long long n = hw * B;
+  const float s2 = s * s;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
+    const long long b = i / hw;
+    const int y = static_cast<int>((i % hw) / w.W), x = static_cast<int>(i % w.W);
+    const long long j = plane_index(w, b, y, x);
+    const float2 c = pc[j];
+    const float a = 1.0f + out[i], q = out[n + i];
+    const float re = c.x * a - c.y * q, im = c.x * q + c.y * a;
+    const float I = (re * re + im * im) * s2, F = tgt[j] * s2;
+    const float g = static_cast<float>(2.0 / ((sums[2 * b + 1] + 1e-300) * B)) * (I - F) * 2.0f * s2;
+    // conj(Psi) Psi_c = (re - i im)(c.x + i c.y)
+    const float pr = re * c.x + im * c.y, pi = re * c.y - im * c.x;
+    gout[i] = g * pr;
+    gout[n + i] = -g * pi;
+  }
+}
+
+// cbed = |Psi_c (1 + o)|^2 * scale inside the window, |Psi_c|^2 * scale outside
+__global__ void spectral_apply_kernel(const float* __restrict__ out, const float2* __restrict__ pc, Window w, int B,
+                                      float scale, float* __restrict__ cbed) {
+  const long long plane = static_cast<long long>(w.Ny) * w.Nx;
+  const long long hw = static_cast<long long>(w.H) * w.W;
+  const long long n = plane * B, nw = hw * B;
+  for (long long j = blockIdx.x * 256LL + threadIdx.x; j < n; j += 256LL * gridDim.x) {
+    const long long b = j / plane;
+    const int y = static_cast<int>((j % plane) / w.Nx) - w.r0, x = static_cast<int>(j % w.Nx) - w.c0;
+    const float2 c = pc[j];
+    float re = c.x, im = c.y;
+    if (y >= 0 && y < w.H && x >= 0 && x < w.W) {
+      const long long i = (b * w.H + y) * w.W + x;
+      const float a = 1.0f + out[i], q = out[nw + i];
+      re = c.x * a - c.y * q;
+      im = c.x * q + c.y * a;
+    }
+    cbed[j] = (re * re + im * im) * scale;
+  }
+}
+
+// out = (scale v (-1)^(y+x) if checker else scale v, 0): a real plane as a
+// complex one whose forward transform comes out centred
+__global__ void real_to_complex_kernel(const float* __restrict__ v, long long n, int Ny, int Nx, float scale,
+                                       int checker, float2* __restrict__ out) {
+  const long long plane = static_cast<long long>(Ny) * Nx;
+  for (long long j = blockIdx.x * 256LL + threadIdx.x; j < n; j += 256LL * gridDim.x) {
+    float a = v[j] * scale;
+    if (checker && ((((j % plane) / Nx) + (j % Nx)) & 1)) a = -a;
+    out[j] = make_float2(a, 0.0f);
+  }
+}
+
 inline int ew_grid(long long n) {
   return static_cast<int>(std::min<long long>((n + 255) / 256, 65536));
+}
+
+int check_window(int B, int Ny, int Nx, int r0, int c0, int H, int W, const char* who) {
+  if (B <= 0 || B > 65535 || Ny <= 0 || Nx <= 0 || H <= 0 || W <= 0 || r0 < 0 || c0 < 0 || r0 + H > Ny ||
+      c0 + W > Nx)
+    return fail(kErrArg, "%s: window [%d,%d)x[%d,%d) outside %dx%d or B=%d", who, r0, r0 + H, c0, c0 + W, Ny, Nx,
+                B);
+  return kOk;
 }
 
 }  // namespace namsa
@@ -254,6 +410,58 @@ int namsa_modes_expand(const float* Ys, float* E, int C, int B, int m, void* str
   const long long n = static_cast<long long>(m) * m * C * B;
   hipLaunchKernelGGL(modes_expand_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream), Ys, E, C, B, m);
   return check_hip(hipGetLastError(), "modes_expand_kernel launch");
+}
+
+int namsa_spectral_pack(const void* psi_c, const void* psi_0, const void* vhat, const int* vidx, int nv, int B, int Ny,
+                        int Nx, int r0, int c0, int H, int W, const float* kmax, float s, float phase_scale,
+                        const float* thick, float thick_scale, float vscale, int opts, float* u, void* stream) {
+  if (int rc = check_window(B, Ny, Nx, r0, c0, H, W, "namsa_spectral_pack")) return rc;
+  if (!psi_c || !psi_0 || !kmax || !thick || !u || ((opts & 1) && (!vhat || nv < 1)) || opts < 0 || opts > 3)
+    return fail(kErrArg, "namsa_spectral_pack: null input or opts=%d", opts);
+  const long long n = static_cast<long long>(B) * H * W;
+  hipLaunchKernelGGL(spectral_pack_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream),
+                     static_cast<const float2*>(psi_c), static_cast<const float2*>(psi_0),
+                     static_cast<const float2*>(vhat), vidx, nv, Window{Ny, Nx, r0, c0, H, W}, B, kmax, s,
+                     phase_scale, thick, thick_scale, vscale, opts, u);
+  return check_hip(hipGetLastError(), "spectral_pack_kernel launch");
+}
+
+int namsa_spectral_loss(const float* out, const void* psi_c, const float* target, int B, int Ny, int Nx, int r0,
+                        int c0, int H, int W, float s, double* sums, float* gout, void* stream) {
+  if (int rc = check_window(B, Ny, Nx, r0, c0, H, W, "namsa_spectral_loss")) return rc;
+  if (!out || !psi_c || !target || !sums) return fail(kErrArg, "namsa_spectral_loss: null argument");
+  hipStream_t st = as_stream(stream);
+  const Window w{Ny, Nx, r0, c0, H, W};
+  const float2* pc = static_cast<const float2*>(psi_c);
+  hipLaunchKernelGGL(zero_doubles_kernel, dim3((2 * B + 255) / 256), dim3(256), 0, st, sums, 2 * B);
+  if (int rc = check_hip(hipGetLastError(), "zero_doubles_kernel launch")) return rc;
+  const long long hw = static_cast<long long>(H) * W;
+  const unsigned chunks = static_cast<unsigned>(std::min<long long>((hw + 255) / 256, 1024));
+  hipLaunchKernelGGL(spectral_sums_kernel, dim3(chunks, B), dim3(256), 0, st, out, pc, target, w, s, sums);
+  if (int rc = check_hip(hipGetLastError(), "spectral_sums_kernel launch")) return rc;
+  if (!gout) return kOk;
+  hipLaunchKernelGGL(spectral_grad_kernel, dim3(ew_grid(hw * B)), dim3(256), 0, st, out, pc, target, w, B, s, sums,
+                     gout);
+  return check_hip(hipGetLastError(), "spectral_grad_kernel launch");
+}
+
+int namsa_spectral_apply(const float* out, const void* psi_c, int B, int Ny, int Nx, int r0, int c0, int H, int W,
+                         float scale, float* cbed, void* stream) {
+  if (int rc = check_window(B, Ny, Nx, r0, c0, H, W, "namsa_spectral_apply")) return rc;
+  if (!out || !psi_c || !cbed) return fail(kErrArg, "namsa_spectral_apply: null argument");
+  const long long n = static_cast<long long>(B) * Ny * Nx;
+  hipLaunchKernelGGL(spectral_apply_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream), out,
+                     static_cast<const float2*>(psi_c), Window{Ny, Nx, r0, c0, H, W}, B, scale, cbed);
+  return check_hip(hipGetLastError(), "spectral_apply_kernel launch");
+}
+
+int namsa_real_to_complex(const float* v, int B, int Ny, int Nx, float scale, int checker, void* out,
+                          void* stream) {
+  if (!v || !out || B <= 0 || Ny <= 0 || Nx <= 0) return fail(kErrArg, "namsa_real_to_complex: bad arguments");
+  const long long n = static_cast<long long>(B) * Ny * Nx;
+  hipLaunchKernelGGL(real_to_complex_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream), v, n, Ny, Nx, scale,
+                     checker ? 1 : 0, static_cast<float2*>(out));
+  return check_hip(hipGetLastError(), "real_to_complex_kernel launch");
 }
 
 int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B, int hw, float s,

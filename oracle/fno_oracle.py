@@ -20,6 +20,11 @@ Network (C channels, L Fourier layers, m x m retained modes, hidden width Hq):
             the stacked real and imaginary parts; a complex C x C weight R
             is the special case [[Re R, -Im R], [Im R, Re R]]).
   gelu(x) = x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))
+
+The spectral (one-shot) corrector's inputs, loss and application are
+restated at the end: spectral_inputs, spectral_loss (pinned by central
+finite differences, the zero-correction value and a closed-form gain) and
+spectral_apply (pinned by its o = 0 and o = -1 special cases).
 """
 from __future__ import annotations
 
@@ -131,3 +136,65 @@ def backward(p, cache, gout):
     g["bl"] = gx.sum(axis=(0, 2, 3))
     g["u"] = np.einsum("cj,bcyx->bjyx", p["Wl"], gx)
     return g
+
+
+# ---------------------------------------------------------------- spectral corrector
+# The one-shot corrector acts on the coarse exit spectrum Psi_c (|Psi|^2 is
+# the CBED) inside the window [r0, r0+H) x [c0, c0+W) that bounds the
+# bandwidth mask, with Psi = Psi_c (1 + o), o = out[0] + i out[1].
+
+def k_linspace(n, kmax):
+    """k_j = j kmax / (n - 1) - kmax / 2 (the linspace k-grid of the probe and propagator)."""
+    return np.arange(n, dtype=np.float64) * kmax / (n - 1) - kmax / 2.0
+
+
+def spectral_inputs(psi_c, psi_0, window, kmax, s, phase_scale, thick, thick_scale, vhat=None, vidx=None,
+                    vscale=0.0, amplitude=False):
+    """Network inputs [B][ch][H][W]: s Psi_c (re, im), s Psi_0 (re, im),
+    phase_scale |k|^2 (on sample b's grid of extent kmax[b]), thick_b thick_scale,
+    then log1p(vscale |vhat|) and |s Psi_c| when given.  psi_c, psi_0:
+    [B][Ny][Nx] complex; vhat: [nv][Ny][Nx]."""
+    r0, c0, H, W = window
+    B, Ny, Nx = psi_c.shape
+    km = np.broadcast_to(np.asarray(kmax, dtype=np.float64), (B,))
+    k2 = np.stack([k_linspace(Ny, k)[r0:r0 + H, None] ** 2 + k_linspace(Nx, k)[None, c0:c0 + W] ** 2 for k in km])
+    win = (slice(None), slice(r0, r0 + H), slice(c0, c0 + W))
+    c = psi_c[win] * s
+    z = psi_0[win] * s
+    ch = [c.real, c.imag, z.real, z.imag, phase_scale * k2,
+          np.broadcast_to((np.asarray(thick, dtype=np.float64) * thick_scale)[:, None, None], c.shape)]
+    if vhat is not None:
+        idx = np.zeros(B, dtype=int) if vidx is None else np.asarray(vidx)
+        ch.append(np.log1p(vscale * np.abs(vhat[idx][win])))
+    if amplitude:
+        ch.append(np.abs(c))
+    return np.stack(ch, axis=1)
+
+
+def spectral_loss(out, psi_c, target, window, s):
+    """mean_b sum (I - F)^2 / sum F^2 with I = |s Psi_c (1 + o)|^2 and F = s^2 target
+    over the window; returns (loss, per-sample (num, den), dloss/dout [B][2][H][W])."""
+    r0, c0, H, W = window
+    win = (slice(None), slice(r0, r0 + H), slice(c0, c0 + W))
+    c = psi_c[win]
+    o = out[:, 0] + 1j * out[:, 1]
+    psi = c * (1.0 + o)
+    I = np.abs(psi) ** 2 * s * s
+    F = target[win] * s * s
+    num = np.sum((I - F) ** 2, axis=(1, 2))
+    den = np.sum(F ** 2, axis=(1, 2))
+    B = out.shape[0]
+    loss = float(np.mean(num / den))
+    g = (2.0 * (I - F) / den[:, None, None] / B) * 2.0 * s * s
+    z = np.conj(psi) * c                         # d|psi|^2/d(o_re) = 2 Re z, d/d(o_im) = -2 Im z
+    grad = np.stack([g * z.real, -g * z.imag], axis=1)
+    return loss, np.stack([num, den], axis=1), grad
+
+
+def spectral_apply(out, psi_c, window, scale=1.0):
+    """CBED: |Psi_c (1 + o)|^2 scale in the window, |Psi_c|^2 scale elsewhere."""
+    r0, c0, H, W = window
+    psi = psi_c.astype(np.complex128).copy()
+    o = out[:, 0] + 1j * out[:, 1]
+    psi[:, r0:r0 + H, c0:c0 + W] *= (1.0 + o)
+    return np.abs(psi) ** 2 * scale

@@ -84,3 +84,72 @@ def test_spectral_layer_matches_fft_with_complex_mode_weights():
         for b, kx in enumerate(F.mode_index(m)):
             Y[:, :, ky % H, kx % W] = np.einsum("oc,bc->bo", R[a, b], U[:, :, ky % H, kx % W])
     np.testing.assert_allclose(cache["pre"][0], np.real(np.fft.ifft2(Y)), atol=1e-10)
+
+
+# ---------------------------------------------------------------- spectral corrector
+def _spectral_case(seed=3):
+    rng = np.random.default_rng(seed)
+    B, Ny, Nx = 3, 12, 14
+    window = (2, 3, 7, 9)
+    pc = rng.standard_normal((B, Ny, Nx)) + 1j * rng.standard_normal((B, Ny, Nx))
+    tgt = np.abs(rng.standard_normal((B, Ny, Nx))) ** 2
+    out = 0.2 * rng.standard_normal((B, 2, 7, 9))
+    return pc, tgt, out, window
+
+
+def test_spectral_loss_gradient_matches_finite_differences():
+    pc, tgt, out, window = _spectral_case()
+    s = 1.7
+    _, _, g = F.spectral_loss(out, pc, tgt, window, s)
+    rng = np.random.default_rng(11)
+    for _ in range(12):
+        idx = tuple(int(rng.integers(n)) for n in out.shape)
+        h = 1e-4                       # the loss is a quartic in o: central differences are exact to O(h^2)
+        op, om = out.copy(), out.copy()
+        op[idx] += h
+        om[idx] -= h
+        num = (F.spectral_loss(op, pc, tgt, window, s)[0] - F.spectral_loss(om, pc, tgt, window, s)[0]) / (2 * h)
+        assert g[idx] == pytest.approx(num, rel=1e-6, abs=1e-9)
+
+
+def test_spectral_loss_special_values():
+    pc, tgt, out, window = _spectral_case()
+    r0, c0, H, W = window
+    win = (slice(None), slice(r0, r0 + H), slice(c0, c0 + W))
+    # no correction: the coarse CBED's relative squared error over the window
+    loss0, sums0, _ = F.spectral_loss(np.zeros_like(out), pc, tgt, window, 1.0)
+    I, T = np.abs(pc[win]) ** 2, tgt[win]
+    expect = np.sum((I - T) ** 2, axis=(1, 2)) / np.sum(T ** 2, axis=(1, 2))
+    np.testing.assert_allclose(sums0[:, 0] / sums0[:, 1], expect, rtol=1e-12)
+    assert loss0 == pytest.approx(expect.mean(), rel=1e-12)
+    # the loss is invariant under a common scale s (relative error)
+    assert F.spectral_loss(out, pc, tgt, window, 3.0)[0] == pytest.approx(F.spectral_loss(out, pc, tgt, window, 1.0)[0],
+                                                                         rel=1e-12)
+    # a real gain o = g - 1 with |g|^2 = T / I makes the CBED exact: loss 0
+    gain = np.sqrt(T / I) - 1.0
+    exact = np.stack([gain, np.zeros_like(gain)], axis=1)
+    assert F.spectral_loss(exact, pc, tgt, window, 1.0)[0] < 1e-28
+
+
+def test_spectral_apply_and_inputs_closed_forms():
+    pc, tgt, out, window = _spectral_case()
+    r0, c0, H, W = window
+    cb0 = F.spectral_apply(np.zeros_like(out), pc, window, scale=2.0)
+    np.testing.assert_allclose(cb0, 2.0 * np.abs(pc) ** 2, rtol=1e-15)
+    minus = np.stack([-np.ones((3, H, W)), np.zeros((3, H, W))], axis=1)      # o = -1 removes the window
+    cb = F.spectral_apply(minus, pc, window)
+    assert np.all(cb[:, r0:r0 + H, c0:c0 + W] == 0)
+    np.testing.assert_allclose(cb[:, :r0], np.abs(pc[:, :r0]) ** 2, rtol=1e-15)
+    # inputs: k^2 on the linspace grid at a hand-picked pixel, thickness channel, vhat channel
+    p0 = pc[::-1].copy()
+    vh = np.ones((2, 12, 14)) * np.array([3.0, 5.0])[:, None, None]
+    u = F.spectral_inputs(pc, p0, window, 8.0, 2.0, 0.5, [40.0, 80.0, 120.0], 0.01, vhat=vh, vidx=[1, 0, 1],
+                          vscale=4.0, amplitude=True)
+    assert u.shape == (3, 8, H, W)
+    ky, kx = 4 * 8.0 / 11 - 4.0, 7 * 8.0 / 13 - 4.0           # pixel (r0 + 2, c0 + 4)
+    assert u[1, 4, 2, 4] == pytest.approx(0.5 * (ky ** 2 + kx ** 2), rel=1e-12)
+    assert u[2, 5, 0, 0] == pytest.approx(1.2)
+    assert u[0, 6, 1, 1] == pytest.approx(np.log1p(20.0)) and u[1, 6, 1, 1] == pytest.approx(np.log1p(12.0))
+    assert u[2, 0, 3, 3] == pytest.approx(2.0 * pc[2, r0 + 3, c0 + 3].real)
+    assert u[2, 3, 3, 3] == pytest.approx(2.0 * p0[2, r0 + 3, c0 + 3].imag)
+    assert u[0, 7, 5, 6] == pytest.approx(2.0 * abs(pc[0, r0 + 5, c0 + 6]))
