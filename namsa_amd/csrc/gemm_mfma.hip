@@ -3,17 +3,26 @@
 //   C[b](i,j) = alpha * sum_k A[b](i,k) B[b](k,j) + beta * C[b](i,j)  (+ bias, GELU)
 //
 // Operands live in HBM as FP32 with arbitrary element strides (so transposes
-// and the planar complex layouts of the corrector network need no copies);
-// a 256-thread workgroup stages a 64x32 tile of A and a 32x64 tile of B
-// through LDS as bf16 (k contiguous), and each of its four wavefronts
-// accumulates a 32x32 block with v_mfma_f32_32x32x16_bf16.  With split_k > 1
-// the K range is divided over blockIdx.z and partial products are added to C
-// with FP32 atomics (C must then already hold beta*C).
+// and the planar complex layouts of the corrector network need no copies).
+// A 256-thread workgroup (4 wavefronts) computes a TM x TN tile of C, each
+// wavefront one 32 x 32 block with v_mfma_f32_32x32x16_bf16.  The tile shape
+// follows the matrix: the corrector's products are skinny (one side is a
+// 32-channel or 32-mode dimension), so N <= 32 takes 128 x 32 tiles, M <= 32
+// takes 32 x 128 tiles and the rest 64 x 64 -- no wavefront idles on a
+// 32-wide side.  Operand tiles of K = 32 are staged through LDS as bf16 with
+// k contiguous: the unit-stride axis is read with 16-byte loads where the
+// strides and the base pointer allow it, and the next K tile is loaded into
+// registers while the matrix cores work on the current one.  MFMA operands
+// come out of LDS as 16-byte reads.  With split_k > 1 the K range is divided
+// over blockIdx.z and partial products are added to C with FP32 atomics (C
+// must then already hold beta*C).
 //
 // MFMA 32x32x16 operand layout (lane l, 8 bf16 per operand register set):
 //   A: row l%32, k = 8*(l/32) + e      B: column l%32, k = 8*(l/32) + e
 //   D: 16 floats, register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "common.hpp"
 #include "namsa_hip.h"
@@ -21,6 +30,7 @@
 namespace namsa {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 struct GemmArgs {
@@ -38,69 +48,130 @@ struct GemmArgs {
   float* P;           // same strides as C
 };
 
-constexpr int kTM = 64, kTN = 64, kTK = 32, kPadK = 8;
+constexpr int kTK = 32;
+constexpr int kLdK = kTK + 8;  // LDS row pitch in bf16: 80 bytes keeps every row 16-byte aligned
 
 __device__ __forceinline__ float gelu_tanh(float x) {
   const float c = 0.7978845608028654f;  // sqrt(2/pi)
   return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
 }
 
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  __shared__ __bf16 As[kTM][kTK + kPadK];
-  __shared__ __bf16 Bs[kTN][kTK + kPadK];  // transposed: [j][k]
+// One operand tile of R rows (i for A, j for B) by kTK k values, element
+// (r, k) at base + (r0 + r) * sr + (k0 + k) * sk, valid for r0 + r < rows
+// and k0 + k < k_end.  Each thread carries G groups of 4 elements that are
+// consecutive along the operand's unit-stride axis (k when kfast).
+template <int R>
+struct Tile {
+  static constexpr int G = R * kTK / 4 / 256;
+  float v[G][4];
 
-  const int tiles_n = (g.N + kTN - 1) / kTN;
+  __device__ __forceinline__ void load(const float* __restrict__ base, long long sr, long long sk, int r0,
+                                       int rows, int k0, int k_end, bool kfast, bool vec) {
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int grp = threadIdx.x + 256 * q;
+      int r, k;
+      if (kfast) {
+        r = grp / (kTK / 4);
+        k = (grp % (kTK / 4)) * 4;
+      } else {
+        k = grp / (R / 4);
+        r = (grp % (R / 4)) * 4;
+      }
+      const int gr = r0 + r, gk = k0 + k;
+      if (vec && (kfast ? (gr < rows && gk + 3 < k_end) : (gr + 3 < rows && gk < k_end))) {
+        const float4 f = *reinterpret_cast<const float4*>(base + gr * sr + gk * sk);
+        v[q][0] = f.x;
+        v[q][1] = f.y;
+        v[q][2] = f.z;
+        v[q][3] = f.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int er = kfast ? gr : gr + e, ek = kfast ? gk + e : gk;
+          v[q][e] = (er < rows && ek < k_end) ? base[er * sr + ek * sk] : 0.0f;
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(__bf16 (*lds)[kLdK], bool kfast) const {
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int grp = threadIdx.x + 256 * q;
+      if (kfast) {
+        const int r = grp / (kTK / 4), k = (grp % (kTK / 4)) * 4;
+        bf16x4 h;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[e] = __bf16(v[q][e]);
+        *reinterpret_cast<bf16x4*>(&lds[r][k]) = h;
+      } else {
+        const int k = grp / (R / 4), r = (grp % (R / 4)) * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) lds[r + e][k] = __bf16(v[q][e]);
+      }
+    }
+  }
+};
+
+// 16-byte loads along the unit-stride axis need that stride to be 1, the
+// other stride and the batch stride to keep 4-float alignment, and an
+// aligned base.
+__device__ __forceinline__ bool vec_ok(const float* base, long long s_unit, long long s_other, long long s_batch) {
+  return s_unit == 1 && (s_other & 3) == 0 && (s_batch & 3) == 0 &&
+         (reinterpret_cast<uintptr_t>(base) & 15) == 0;
+}
+
+template <int WM, int WN>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+  constexpr int TM = 32 * WM, TN = 32 * WN;
+  __shared__ __align__(16) __bf16 As[TM][kLdK];
+  __shared__ __align__(16) __bf16 Bs[TN][kLdK];  // transposed: [j][k]
+
+  const int tiles_n = (g.N + TN - 1) / TN;
   const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
   const int b = blockIdx.y;
   const int split = blockIdx.z;
-  const int i0 = tm * kTM, j0 = tn * kTN;
+  const int i0 = tm * TM, j0 = tn * TN;
   const int k_per = ((g.K + g.split_k - 1) / g.split_k + kTK - 1) / kTK * kTK;
   const int k_begin = split * k_per;
   const int k_end = min(g.K, k_begin + k_per);
 
   const float* A = g.A + b * g.sAb;
   const float* B = g.B + b * g.sBb;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave / WN, wn = wave % WN;
 
-  // staging map: 8 elements per thread per tile, contiguous along the
-  // operand's unit-stride axis so global reads coalesce
-  const bool a_kfast = g.sAk <= g.sAi;
-  const bool b_kfast = g.sBk <= g.sBj;
+  // A rows are i (stride sAi), B rows are j (stride sBj); k is the column
+  const bool a_kfast = g.sAk == 1 || (g.sAi != 1 && g.sAk <= g.sAi);
+  const bool b_kfast = g.sBk == 1 || (g.sBj != 1 && g.sBk <= g.sBj);
+  const bool a_vec = a_kfast ? vec_ok(g.A, g.sAk, g.sAi, g.sAb) : vec_ok(g.A, g.sAi, g.sAk, g.sAb);
+  const bool b_vec = b_kfast ? vec_ok(g.B, g.sBk, g.sBj, g.sBb) : vec_ok(g.B, g.sBj, g.sBk, g.sBb);
 
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
 
+  Tile<TM> ta;
+  Tile<TN> tb;
+  if (k_begin < k_end) {
+    ta.load(A, g.sAi, g.sAk, i0, g.M, k_begin, k_end, a_kfast, a_vec);
+    tb.load(B, g.sBj, g.sBk, j0, g.N, k_begin, k_end, b_kfast, b_vec);
+  }
   for (int k0 = k_begin; k0 < k_end; k0 += kTK) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = tid + 256 * e;  // 0..2047 over the 64x32 tile
-      int ii, kk;
-      if (a_kfast) { ii = idx / kTK; kk = idx % kTK; } else { ii = idx % kTM; kk = idx / kTM; }
-      const int gi = i0 + ii, gk = k0 + kk;
-      float v = 0.0f;
-      if (gi < g.M && gk < k_end) v = A[gi * g.sAi + gk * g.sAk];
-      As[ii][kk] = __bf16(v);
-      int jj, kb;
-      if (b_kfast) { jj = idx / kTK; kb = idx % kTK; } else { jj = idx % kTN; kb = idx / kTN; }
-      const int gj = j0 + jj, gkb = k0 + kb;
-      float w = 0.0f;
-      if (gj < g.N && gkb < k_end) w = B[gkb * g.sBk + gj * g.sBj];
-      Bs[jj][kb] = __bf16(w);
-    }
+    ta.store(As, a_kfast);
+    tb.store(Bs, b_kfast);
     __syncthreads();
+    if (k0 + kTK < k_end) {  // next K tile in flight while the matrix cores work
+      ta.load(A, g.sAi, g.sAk, i0, g.M, k0 + kTK, k_end, a_kfast, a_vec);
+      tb.load(B, g.sBj, g.sBk, j0, g.N, k0 + kTK, k_end, b_kfast, b_vec);
+    }
+    const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31);
 #pragma unroll
     for (int ks = 0; ks < kTK; ks += 16) {
-      bf16x8 a, bb;
-      const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31);
       const int kk = ks + 8 * (lane >> 5);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        a[e] = As[ra][kk + e];
-        bb[e] = Bs[rb][kk + e];
-      }
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[ra][kk]);
+      const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&Bs[rb][kk]);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
     }
     __syncthreads();
@@ -130,6 +201,16 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
+template <int WM, int WN>
+int launch_gemm(const GemmArgs& g, hipStream_t st) {
+  constexpr int TM = 32 * WM, TN = 32 * WN;
+  const long long tiles = static_cast<long long>((g.M + TM - 1) / TM) * ((g.N + TN - 1) / TN);
+  if (tiles > 0x7fffffffLL) return fail(kErrArg, "namsa_gemm_bf16: too many tiles");
+  hipLaunchKernelGGL((gemm_bf16_kernel<WM, WN>), dim3(static_cast<unsigned>(tiles), g.batch, g.split_k), dim3(256), 0,
+                     st, g);
+  return check_hip(hipGetLastError(), "gemm_bf16_kernel launch");
+}
+
 }  // namespace namsa
 
 using namespace namsa;
@@ -151,11 +232,10 @@ int namsa_gemm_bf16(const float* A, long long sAb, long long sAi, long long sAk,
     return fail(kErrArg, "namsa_gemm_bf16: missing bias or pre-activation buffer");
   GemmArgs g{A, sAb, sAi, sAk, B, sBb, sBk, sBj, C, sCb, sCi, sCj, M, N, K, batch, split_k,
              alpha, beta, bias, bias_mode, gelu, P};
-  const long long tiles = static_cast<long long>((M + kTM - 1) / kTM) * ((N + kTN - 1) / kTN);
-  if (tiles > 0x7fffffffLL) return fail(kErrArg, "namsa_gemm_bf16: too many tiles");
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(static_cast<unsigned>(tiles), batch, split_k), dim3(256),
-                     0, as_stream(stream), g);
-  return check_hip(hipGetLastError(), "gemm_bf16_kernel launch");
+  hipStream_t st = as_stream(stream);
+  if (N <= 32) return launch_gemm<4, 1>(g, st);
+  if (M <= 32) return launch_gemm<1, 4>(g, st);
+  return launch_gemm<2, 2>(g, st);
 }
 
 }  // extern "C"

@@ -72,6 +72,41 @@ def test_gemm_layouts(gpu, M, N, K, batch, a_t, b_t):
     assert C[M * N] == 7.0
 
 
+@pytest.mark.parametrize("M,N,K,batch,sA,sB,offA,offB", [
+    (300, 32, 200, 2, (300 * 200, 200, 1), (200 * 32, 32, 1), 0, 0),    # 128x32 tiles, 16-byte loads of A and B
+    (300, 17, 70, 2, (300 * 72, 72, 1), (70 * 20, 20, 1), 0, 0),        # ragged N and K with padded strides
+    (32, 1000, 32, 2, (32 * 32, 32, 1), (32 * 1000, 1000, 1), 0, 0),    # 32x128 tiles (a pointwise layer)
+    (7, 300, 64, 1, (7 * 64, 64, 1), (64 * 300, 300, 1), 0, 0),         # ragged M below one tile
+    (128, 200, 96, 2, (128 * 96, 1, 128), (96 * 200, 200, 1), 0, 0),    # A unit stride along i
+    (64, 96, 40, 1, (64 * 40, 40, 1), (40 * 96, 1, 40), 0, 0),          # B unit stride along k
+    (300, 32, 200, 1, (300 * 200, 200, 1), (200 * 32, 32, 1), 1, 3),    # misaligned bases: element loads
+])
+def test_gemm_tile_shapes_and_vector_loads(gpu, M, N, K, batch, sA, sB, offA, offB):
+    import torch
+    _say(f"gemm tiles M={M} N={N} K={K} batch={batch} offsets=({offA},{offB})")
+    rng = np.random.default_rng(M + 3 * N + 7 * K)
+    sC = (M * N, N, 1)
+    A = rng.standard_normal(offA + batch * sA[0]).astype(np.float32)
+    B = rng.standard_normal(offB + batch * sB[0]).astype(np.float32)
+    Ad, Bd = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    Cd = torch.zeros(batch * sC[0], dtype=torch.float32, device="cuda")
+    call("namsa_gemm_bf16", C_ptr(Ad, offA), *sA, C_ptr(Bd, offB), *sB, ptr(Cd), *sC, M, N, K, batch, 1,
+         np.float32(1.0), np.float32(0.0), None, 0, 0, None, _lib.stream_handle())
+    torch.cuda.synchronize()
+    C = Cd.cpu().numpy()
+    Ab, Bb = bf16(A[offA:]), bf16(B[offB:])
+    for b in range(batch):
+        ref = _view(Ab, b, sA, M, K) @ _view(Bb, b, sB, K, N)
+        err = np.linalg.norm(_view(C, b, sC, M, N) - ref) / np.linalg.norm(ref)
+        assert err < 2e-5, (b, err)
+
+
+def C_ptr(t, off):
+    """Device pointer `off` floats into tensor t."""
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr() + 4 * off)
+
+
 def test_gemm_epilogue_bias_beta_gelu_and_split_k(gpu):
     import torch
     _say("gemm epilogues")
@@ -118,7 +153,7 @@ def test_gemm_rejects_bad_arguments(gpu):
 
 def test_pointwise_kernels(gpu):
     import torch
-    _say("fill / gelu backward / adam / pack / loss / apply / intensity")
+    _say("fill / gelu backward / adam")
     rng = np.random.default_rng(9)
     n = 5000
     x = torch.empty(n, device="cuda")
@@ -127,7 +162,8 @@ def test_pointwise_kernels(gpu):
     pre = rng.standard_normal(n).astype(np.float32) * 3
     g = rng.standard_normal(n).astype(np.float32)
     gd = torch.from_numpy(g.copy()).cuda()
-    call("namsa_gelu_backward", ptr(gd), ptr(torch.from_numpy(pre).cuda()), n, _lib.stream_handle())
+    pre_d = torch.from_numpy(pre).cuda()
+    call("namsa_gelu_backward", ptr(gd), ptr(pre_d), n, _lib.stream_handle())
     # fp32 tanh: absolute error ~1e-6 where gelu' passes through zero near x = -0.75 and x < -3
     np.testing.assert_allclose(gd.cpu().numpy(), g * F.gelu_grad(pre.astype(np.float64)), rtol=2e-5, atol=1e-5)
     # Adam, two steps, against the textbook update with decoupled decay
@@ -137,67 +173,13 @@ def test_pointwise_kernels(gpu):
     p, m, v = p0.astype(np.float64), np.zeros(n), np.zeros(n)
     lr, b1, b2, eps, wd = 1e-2, 0.9, 0.999, 1e-8, 1e-3
     for step, gr in enumerate(grads, 1):
-        call("namsa_adam", ptr(pd), ptr(torch.from_numpy(gr).cuda()), ptr(md), ptr(vd), n, np.float32(lr),
+        gr_d = torch.from_numpy(gr).cuda()
+        call("namsa_adam", ptr(pd), ptr(gr_d), ptr(md), ptr(vd), n, np.float32(lr),
              np.float32(b1), np.float32(b2), np.float32(eps), step, np.float32(wd), _lib.stream_handle())
         m = b1 * m + (1 - b1) * gr
         v = b2 * v + (1 - b2) * gr.astype(np.float64) ** 2
         p = p - lr * ((m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps) + wd * p)
     np.testing.assert_allclose(pd.cpu().numpy(), p, rtol=1e-5, atol=1e-6)
-    # pack / loss / apply on B = 3 samples of hw = 40 with nv = 2 potential channels
-    B, hw, nv, s, sv = 3, 40, 2, 4.0, 0.5
-    pc = (rng.standard_normal((B, hw)) + 1j * rng.standard_normal((B, hw))).astype(np.complex64)
-    pf = (pc + 0.1 * (rng.standard_normal((B, hw)) + 1j * rng.standard_normal((B, hw)))).astype(np.complex64)
-    V = rng.standard_normal((B, nv, hw)).astype(np.float32)
-    pcd, pfd = torch.from_numpy(pc).cuda(), torch.from_numpy(pf).cuda()
-    u = torch.empty((2 + nv, B * hw), device="cuda")
-    Ny, Nx = 5, 8                                   # hw = 40
-    call("namsa_corrector_pack", ptr(pcd), ptr(torch.from_numpy(V).cuda()), nv, 0, 0, ptr(u), B, Ny, Nx,
-         np.float32(s), np.float32(sv), _lib.stream_handle())
-    un = u.cpu().numpy()
-    np.testing.assert_allclose(un[0], pc.real.ravel() * s, rtol=1e-6)
-    np.testing.assert_allclose(un[1], pc.imag.ravel() * s, rtol=1e-6)
-    np.testing.assert_allclose(un[2:], V.transpose(1, 0, 2).reshape(nv, -1) * sv, rtol=1e-6)
-    call("namsa_corrector_pack", ptr(pcd), ptr(torch.from_numpy(V[0].copy()).cuda()), nv, 1, 0, ptr(u), B, Ny,
-         Nx, np.float32(s), np.float32(sv), _lib.stream_handle())
-    np.testing.assert_allclose(u.cpu().numpy()[2:], np.tile(V[0], (1, B)) * sv, rtol=1e-6)
-    # with derivative channels: periodic Laplacian and central differences per sample plane
-    u8 = torch.empty((8 + nv, B * hw), device="cuda")
-    call("namsa_corrector_pack", ptr(pcd), ptr(torch.from_numpy(V).cuda()), nv, 0, 1, ptr(u8), B, Ny, Nx,
-         np.float32(s), np.float32(sv), _lib.stream_handle())
-    w = pc.astype(np.complex128).reshape(B, Ny, Nx) * s
-    r = lambda a, d, ax: np.roll(a, d, axis=ax)
-    lap = r(w, 1, 1) + r(w, -1, 1) + r(w, 1, 2) + r(w, -1, 2) - 4 * w
-    dy = 0.5 * (r(w, -1, 1) - r(w, 1, 1))
-    dx = 0.5 * (r(w, -1, 2) - r(w, 1, 2))
-    got = u8.cpu().numpy()
-    for c, f in enumerate((w, lap, dy, dx)):
-        np.testing.assert_allclose(got[2 * c], f.real.ravel(), rtol=1e-5, atol=1e-5)
-        np.testing.assert_allclose(got[2 * c + 1], f.imag.ravel(), rtol=1e-5, atol=1e-5)
-    np.testing.assert_allclose(got[8:], V.transpose(1, 0, 2).reshape(nv, -1) * sv, rtol=1e-6)
-    out = rng.standard_normal((2, B * hw)).astype(np.float32)
-    outd = torch.from_numpy(out).cuda()
-    sums = torch.empty(2 * B, dtype=torch.float64, device="cuda")
-    gout = torch.empty((2, B * hw), device="cuda")
-    call("namsa_corrector_loss", ptr(outd), ptr(pcd), ptr(pfd), B, hw, np.float32(s), ptr(sums), ptr(gout),
-         _lib.stream_handle())
-    d = ((pf.astype(np.complex128) - pc) * s).reshape(-1)
-    o = out[0].astype(np.float64) + 1j * out[1]
-    r = (o - d).reshape(B, hw)
-    num = np.sum(np.abs(r) ** 2, axis=1)
-    den = np.sum(np.abs(d.reshape(B, hw)) ** 2, axis=1)
-    np.testing.assert_allclose(sums.cpu().numpy().reshape(B, 2), np.stack([num, den], 1), rtol=1e-5)
-    gref = (r * (2 / (den[:, None] * B))).reshape(-1)
-    np.testing.assert_allclose(gout.cpu().numpy(), np.stack([gref.real, gref.imag]), rtol=1e-4, atol=1e-7)
-    psi = torch.empty_like(pcd)
-    call("namsa_corrector_apply", ptr(outd), ptr(pcd), ptr(psi), B, hw, np.float32(s), _lib.stream_handle())
-    np.testing.assert_allclose(psi.cpu().numpy().ravel(), pc.ravel() + o / s, rtol=1e-5, atol=1e-6)
-    I = torch.empty(B * hw, device="cuda")
-    call("namsa_intensity", ptr(pcd), B * hw, hw, np.float32(0.25), None, ptr(I), _lib.stream_handle())
-    np.testing.assert_allclose(I.cpu().numpy(), np.abs(pc.ravel()) ** 2 * 0.25, rtol=1e-5)
-    mask = (rng.random(hw) > 0.5).astype(np.float32)
-    call("namsa_intensity", ptr(pcd), B * hw, hw, np.float32(0.25), ptr(torch.from_numpy(mask).cuda()), ptr(I),
-         _lib.stream_handle())
-    np.testing.assert_allclose(I.cpu().numpy(), np.abs(pc.ravel()) ** 2 * 0.25 * np.tile(mask, B), rtol=1e-5)
 
 
 def test_mode_bookkeeping(gpu):
