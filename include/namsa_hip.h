@@ -311,19 +311,6 @@ int namsa_gelu_backward(float* g, const float* pre, long long n, void* stream);
 /* Adam with decoupled weight decay; step counts from 1 */
 int namsa_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
                float eps, int step, float wd, void* stream);
-/* u[cin][B*Ny*Nx] = (re, im) psi_c * s; with derivs also (re, im) of the
- * periodic 5-point Laplacian, central y and central x differences of
- * psi_c * s (6 channels); then V_j * sv for j < nv.  cin = 2 + 6 derivs + nv.
- * V is [B][nv][Ny*Nx], or [nv][Ny*Nx] shared by every sample if v_shared. */
-int namsa_corrector_pack(const void* psi_c, const float* V, int nv, int v_shared, int derivs, float* u,
-                         int B, int Ny, int Nx, float s, float sv, void* stream);
-/* out[i] = |F[i]|^2 * scale (* mask[i % hw] if mask) over n complex values */
-int namsa_intensity(const void* F, long long n, long long hw, float scale, const float* mask, float* out,
-                    void* stream);
-/* per sample: sums[2b] = |out - (psi_f - psi_c) s|^2, sums[2b+1] = |(psi_f - psi_c) s|^2;
- * gout (optional) = gradient of mean_b sums[2b] / sums[2b+1] */
-int namsa_corrector_loss(const float* out, const void* psi_c, const void* psi_f, int B, int hw, float s,
-                         double* sums, float* gout, void* stream);
 /*
  * Mode bookkeeping of the spectral layers (m retained modes per axis, C
  * channels, B samples).  combine: per (c,b) the 2m x 2m product
@@ -338,10 +325,6 @@ int namsa_corrector_loss(const float* out, const void* psi_c, const void* psi_f,
  */
 int namsa_modes_combine(const float* P, float* Xs, int C, int B, int m, void* stream);
 int namsa_modes_expand(const float* Ys, float* E, int C, int B, int m, void* stream);
-/* psi = psi_c + out / s (out is [2][B*hw]) */
-int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B, int hw, float s,
-                          void* stream);
-
 /*
  * Spectral (one-shot) corrector: works on the centred exit spectrum Psi(k) of
  * the coarse path, |Psi|^2 being the CBED, inside the window

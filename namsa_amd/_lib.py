@@ -59,10 +59,6 @@ SIGNATURES = {
     "namsa_fill": (_i, [_vp, _ll, _f, _vp]),
     "namsa_gelu_backward": (_i, [_vp, _vp, _ll, _vp]),
     "namsa_adam": (_i, [_vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _i, _f, _vp]),
-    "namsa_corrector_pack": (_i, [_vp, _vp, _i, _i, _i, _vp, _i, _i, _i, _f, _f, _vp]),
-    "namsa_intensity": (_i, [_vp, _ll, _ll, _f, _vp, _vp, _vp]),
-    "namsa_corrector_loss": (_i, [_vp, _vp, _vp, _i, _i, _f, _vp, _vp, _vp]),
-    "namsa_corrector_apply": (_i, [_vp, _vp, _vp, _i, _i, _f, _vp]),
     "namsa_row_fft": (_i, [_vp, _i, _i, _i, _i, _f, _vp]),
     "namsa_precise_slices": (_i, [_i]),
     "namsa_sched_precise": (_i, [_i, _i, _i]),

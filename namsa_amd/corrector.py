@@ -1,40 +1,30 @@
 """Learned coarse-slice corrector on MI355X (no counterpart in the reference).
 
 A thick specimen costs one fused slice step per dz = 1 A of thickness.  The
-corrector takes coarse steps of `factor` (8) fine slices -- one transmission
-exp(i sigma V_coarse) and one propagator over factor*dz -- and corrects the
-wave after each coarse step with a Fourier neural operator trained on pairs
-(coarse step, `factor` fine steps) from the same incoming wave:
+learned form takes coarse steps of f fine slices (f = 8: one transmission
+exp(i sigma V_coarse) and one propagator over 8 A) and corrects each probe's
+result once, at the exit: the network maps the coarse exit spectrum Psi_c
+(the complex amplitude whose squared modulus is the CBED) to a
+multiplicative correction, Psi = Psi_c (1 + o), trained against the fine
+path's CBEDs on pairs the plain method generates (SpectralMaterial,
+SpectralPairPool, make_spectral_pools; namsa_amd.train_corrector).  The
+correction costs one network pass per probe whatever the thickness, so it is
+amortised over all the coarse slices it replaces fine ones for.
 
-    input  u = (Re, Im of s x_c, s Lap x_c, s d_y x_c, s d_x x_c, sigma V_0 .. sigma V_{factor-1})
-    output Delta (2 channels):  x <- x_c + Delta / s
-
-x_c is the real-space wave after the coarse step (H x W), Lap and d_y, d_x
-its periodic 5-point Laplacian and central differences (the splitting error
-of a coarse step is, to first order, a sum of sub-slice potentials times
-derivatives of the wave, and 1x1 layers cannot form derivatives), V_j the
-projected potentials of the fine slices the coarse slice merges (the network
-needs their order along z: that ordering is exactly what the single coarse
-transmission loses) and s = Nx sqrt(H W / norm) puts the wave at unit RMS.
-
-Network (the FP64 definition and hand-derived backward live in the oracle and
-are pinned there by finite differences): lift (2+factor -> C), L Fourier
-layers  x <- gelu(Re IDFT(G(k) [Re; Im] DFT(x)) + W x + b)  over m x m centred
-modes with a real 2C x 2C weight per mode, and a 2-layer pointwise head.
-Every product -- the truncated DFTs along y and x, the per-mode channel
-mixing, the pointwise layers and all weight gradients -- is one call of the
-BF16 MFMA GEMM (namsa_gemm_bf16, FP32 accumulation); GELU and biases run in
-its epilogue, the rest in the pointwise kernels of the same library.
+Network (FnoNet; the FP64 definition and hand-derived backward live in the
+oracle and are pinned there by finite differences): lift (cin -> C), L
+Fourier layers  x <- gelu(Re IDFT(G(k) [Re; Im] DFT(x)) + W x + b)  over
+m x m centred modes with a real 2C x 2C weight per mode, and a 2-layer
+pointwise head.  Every product -- the truncated DFTs along y and x, the
+per-mode channel mixing, the pointwise layers and all weight gradients -- is
+one call of the BF16 MFMA GEMM (namsa_gemm_bf16, FP32 accumulation); GELU
+and biases run in its epilogue, the rest in the pointwise kernels of the
+same library.
 
 Data layout (HBM, FP32): activations [C][B*H*W] (channel-major, so a
 pointwise layer is one [C x C] x [C x n] GEMM and the x-DFT is one
 [(C B H) x W] x [W x 2m] GEMM); spectral coefficients [m*m][2C][B] (mode-major,
 so the mixing is one batched [2C x 2C] x [2C x B] GEMM over the m*m modes).
-
-Fallback: the predicted correction's relative size |Delta|^2 / |s x_c|^2 is a
-free by-product of the forward pass; when it exceeds `tau` for any probe of
-the batch, the coarse slice is redone with `factor` fine steps from the saved
-incoming state.
 """
 from __future__ import annotations
 
@@ -58,12 +48,11 @@ def _addr(t, off=0):
 class FnoNet:
     """Parameters, forward and backward of the corrector network on the GPU."""
 
-    def __init__(self, H, W, cin=16, C=32, L=4, m=16, Hq=64, seed=0, device="cuda", derivs=True):
+    def __init__(self, H, W, cin=7, C=32, L=4, m=16, Hq=64, seed=0, device="cuda"):
         torch = _torch()
         if m % 2 or m > min(H, W):
             raise ValueError("m must be even and at most the grid size")
         self.H, self.W, self.cin, self.C, self.L, self.m, self.Hq = H, W, cin, C, L, m, Hq
-        self.derivs = bool(derivs)      # inputs carry the wave's Laplacian and gradient
         self.device = torch.device(device)
         self.shapes = {
             "Wl": (C, cin), "bl": (C,), "G": (L, m, m, 2 * C, 2 * C), "W": (L, C, C), "b": (L, C),
@@ -126,8 +115,7 @@ class FnoNet:
         return {k: self.gview(k).cpu().numpy().astype(np.float64) for k in self.shapes}
 
     def state_dict(self):
-        return {"config": dict(H=self.H, W=self.W, cin=self.cin, C=self.C, L=self.L, m=self.m, Hq=self.Hq,
-                               derivs=self.derivs),
+        return {"config": dict(H=self.H, W=self.W, cin=self.cin, C=self.C, L=self.L, m=self.m, Hq=self.Hq),
                 "theta": self.theta.detach().cpu()}
 
     def _p(self, name, idx=0):
@@ -287,263 +275,6 @@ class FnoNet:
         return 2 * n * C * self.cin + L * (spec + 2 * n * C * C) + 2 * n * (Hq * C + 2 * Hq)
 
 
-# ---------------------------------------------------------------------- material
-class CoarseFineMaterial:
-    """One structure prepared for fine (dz) and coarse (factor*dz) stepping.
-
-    Fine slices t_f[S_f], their projected potentials V_f[S_f] (float32), coarse
-    slices t_c[S_c] (the same site table binned factor*dz thick, so coarse
-    slice k merges fine slices factor*k .. factor*k + factor - 1) and both
-    propagators.
-    """
-
-    def __init__(self, sim, dz=1.0, factor=8, bandwidth=1 / 3, probe_dict=None):
-        torch = _torch()
-        self.sim = sim
-        self.factor = int(factor)
-        sim._check_grid()
-        if not hasattr(sim, "tables_device"):
-            sim.calc_atomic_potentials()
-        Ny, Nx = sim._grid
-        self.Ny, self.Nx = Ny, Nx
-        sim.build_potential_slices(dz)
-        self.t_f = sim.potential_slices_device
-        self.S_f = int(sim.num_slices)
-        sites, offs = sim.site_table(dz)
-        Py, Px = sim.table_shape
-        self.V_f = torch.empty((self.S_f, Ny, Nx), dtype=torch.float32, device=sim.device)
-        sites_d = torch.from_numpy(sites.reshape(-1)).to(sim.device) if len(sites) else None
-        call("namsa_build_potential", ptr(sim.tables_device), len(sim.unique_Z), Py, Px, ptr(sites_d),
-             ptr(torch.from_numpy(offs).to(sim.device)), self.S_f, Ny, Nx, ptr(self.V_f), stream_handle())
-        sim.slice_t = float(dz)
-        self.pm_f = sim.build_propagator(bandwidth).clone()
-        sim.build_potential_slices(dz * factor)
-        self.t_c = sim.potential_slices_device
-        self.S_c = int(sim.num_slices)
-        if self.S_c * factor > self.S_f:
-            raise NamsaError("coarse slices extend past the fine ones")
-        sim.slice_t = float(dz * factor)
-        self.pm_c = sim.build_propagator(bandwidth).clone()
-        self.mask = sim.mask_device.clone()
-        self.band = sim.band
-        if not hasattr(sim, "psi_k_device") or probe_dict is not None:
-            sim.build_probe(probe_dict) if probe_dict is not None else sim.build_probe()
-        self.norm = float(sim.normalization)
-        self.s = float(Nx * math.sqrt(Ny * Nx / self.norm))
-        self.sv = float(sim.sigma)
-
-    # state helpers (B planes of the row-transformed wave)
-    def init(self, ws, pos):
-        B = ws.shape[0]
-        call("namsa_probe_init", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
-             np.float32(self.sim.kmax), ptr(ws), stream_handle())
-
-    def step(self, ws, t, pm):
-        B, st = ws.shape[0], stream_handle()
-        call("namsa_slice_rows", ptr(ws), B, self.Ny, self.Nx, ptr(t), 0, self.Nx, st)
-        call("namsa_slice_cols", ptr(ws), B, self.Ny, self.Nx, ptr(pm), 0, self.Nx, st)
-
-    def fine_steps(self, ws, k):
-        for j in range(self.factor):
-            self.step(ws, self.t_f[self.factor * k + j], self.pm_f)
-
-    def to_real(self, ws):
-        call("namsa_row_fft", ptr(ws), ws.shape[0], self.Ny, self.Nx, 1, np.float32(1.0 / self.Nx),
-             stream_handle())
-
-    def to_state(self, x):
-        call("namsa_row_fft", ptr(x), x.shape[0], self.Ny, self.Nx, 0, np.float32(1.0), stream_handle())
-
-    def cbed_from_real(self, x, cbed):
-        """CBED of real-space waves x (destroyed): |FFT2 x|^2 Nx^2 / (Ny Nx norm)
-        inside the bandwidth mask (every propagator already zeroes the rest)."""
-        B = x.shape[0]
-        hw = self.Ny * self.Nx
-        call("namsa_fft2d", ptr(x), B, self.Ny, self.Nx, 0, stream_handle())
-        call("namsa_intensity", ptr(x), B * hw, hw, np.float32(self.Nx / (self.Ny * self.norm)),
-             ptr(self.mask), ptr(cbed), stream_handle())
-
-    def n_inputs(self, derivs):
-        return 2 + (6 if derivs else 0) + self.factor
-
-    def pairs(self, pos):
-        """Yield (k, x_coarse, x_fine) for every coarse slice: both real-space
-        waves after coarse slice k, from the same fine-stepped incoming wave."""
-        torch = _torch()
-        B = pos.shape[0]
-        ws = torch.empty((B, self.Ny, self.Nx), dtype=torch.complex64, device=self.sim.device)
-        self.init(ws, pos)
-        for k in range(self.S_c):
-            xc = ws.clone()
-            self.step(xc, self.t_c[k], self.pm_c)
-            self.fine_steps(ws, k)
-            xf = ws.clone()
-            self.to_real(xc)
-            self.to_real(xf)
-            yield k, xc, xf
-
-    def potentials(self, k):
-        return self.V_f[self.factor * k:self.factor * (k + 1)]
-
-
-# ---------------------------------------------------------------------- inference
-def run_plain(mat, pos, coarse=False):
-    """CBEDs of the plain path over the same thickness, fine (factor*S_c
-    slices) or coarse (S_c slices), through the fused batched driver."""
-    torch = _torch()
-    B = pos.shape[0]
-    dev = mat.sim.device
-    ws = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.complex64, device=dev)
-    cbed = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.float32, device=dev)
-    t, S, pm = (mat.t_c, mat.S_c, mat.pm_c) if coarse else (mat.t_f, mat.factor * mat.S_c, mat.pm_f)
-    call("namsa_multislice", ptr(mat.sim.psi_k_device), ptr(pos), B, mat.Ny, mat.Nx, np.float32(mat.sim.kmax),
-         ptr(t), S, ptr(pm), mat.band[0], mat.band[1], B, _lib.OUT_CBED, np.float32(1.0 / mat.norm), ptr(ws),
-         ptr(cbed), None, 0, None, None, stream_handle())
-    return cbed
-
-
-def run_learned(net, mat, pos, tau=None):
-    """CBEDs through S_c coarse steps, each corrected by the network.
-
-    tau: fallback threshold on the predicted relative correction
-    |Delta|^2 / |s x_c|^2 (max over the batch); a coarse slice above it is
-    redone with fine steps.  Returns (cbed, number of fallback slices)."""
-    torch = _torch()
-    if net.cin != mat.n_inputs(net.derivs):
-        raise NamsaError("network input channels do not match the coarsening factor")
-    B = pos.shape[0]
-    hw = mat.Ny * mat.Nx
-    st = stream_handle()
-    ws = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.complex64, device=mat.sim.device)
-    x = torch.empty_like(ws)
-    keep = torch.empty_like(ws) if tau is not None else None
-    net.ensure(B)
-    mat.init(ws, pos)
-    fallbacks = 0
-    for k in range(mat.S_c):
-        if keep is not None:
-            keep.copy_(ws)
-        mat.step(ws, mat.t_c[k], mat.pm_c)
-        x.copy_(ws)
-        mat.to_real(x)
-        call("namsa_corrector_pack", ptr(x), ptr(mat.potentials(k)), mat.factor, 1, int(net.derivs), ptr(net.u),
-             B, mat.Ny, mat.Nx, np.float32(mat.s), np.float32(mat.sv), st)
-        net.forward(B)
-        if keep is not None:
-            call("namsa_corrector_loss", ptr(net.out), ptr(x), ptr(x), B, hw, np.float32(mat.s), ptr(net.sums),
-                 None, st)
-            pred = net.sums[0:2 * B:2].max().item() / hw
-            if pred > tau:
-                ws.copy_(keep)
-                mat.fine_steps(ws, k)
-                fallbacks += 1
-                continue
-        call("namsa_corrector_apply", ptr(net.out), ptr(x), ptr(ws), B, hw, np.float32(mat.s), st)
-        mat.to_state(ws)
-    mat.to_real(ws)
-    cbed = torch.empty((B, mat.Ny, mat.Nx), dtype=torch.float32, device=ws.device)
-    mat.cbed_from_real(ws, cbed)
-    return cbed, fallbacks
-
-
-# ---------------------------------------------------------------------- training
-class PairPool:
-    """Training pairs resident in HBM: coarse and fine real-space waves plus
-    a (material, coarse slice) index for the potentials."""
-
-    def __init__(self):
-        self.xc, self.xf, self.ref = [], [], []
-        self.mats = []
-
-    def add_material(self, mat, pos):
-        """Pairs of every coarse slice for probes at pos, stored at unit RMS
-        (times the material's s) so materials with different probe norms mix."""
-        if self.mats and abs(mat.sv / self.mats[0].sv - 1) > 1e-6:
-            raise NamsaError("materials of one pool must share the beam energy")
-        mi = len(self.mats)
-        self.mats.append(mat)
-        for k, xc, xf in mat.pairs(pos):
-            xc.mul_(mat.s)
-            xf.mul_(mat.s)
-            for b in range(xc.shape[0]):
-                self.xc.append(xc[b])
-                self.xf.append(xf[b])
-                self.ref.append((mi, k))
-
-    def finalize(self):
-        torch = _torch()
-        self.XC = torch.stack(self.xc)
-        self.XF = torch.stack(self.xf)
-        self.xc, self.xf = None, None
-        return self
-
-    def __len__(self):
-        return len(self.ref)
-
-    def batch(self, idx, V_out):
-        """Gather samples idx into contiguous (xc, xf) and V_out [B][factor][H][W]."""
-        torch = _torch()
-        ii = torch.as_tensor(np.asarray(idx), device=self.XC.device)
-        for j, i in enumerate(idx):
-            mi, k = self.ref[i]
-            V_out[j].copy_(self.mats[mi].potentials(k))
-        return self.XC.index_select(0, ii), self.XF.index_select(0, ii)
-
-
-def batch_loss(net, pool, idx, V, grad=False):
-    """Per-sample relative residual |out - Delta|^2 / |Delta|^2 of samples idx
-    (and, with grad, the gradient of its mean in net.grad)."""
-    mat = pool.mats[pool.ref[idx[0]][0]]
-    B = len(idx)
-    hw = mat.Ny * mat.Nx
-    st = stream_handle()
-    xc, xf = pool.batch(idx, V[:B])
-    one = np.float32(1.0)      # pool waves are stored already scaled by s
-    call("namsa_corrector_pack", ptr(xc), ptr(V), mat.factor, 0, int(net.derivs), ptr(net.u), B, mat.Ny,
-         mat.Nx, one, np.float32(mat.sv), st)
-    net.forward(B)
-    call("namsa_corrector_loss", ptr(net.out), ptr(xc), ptr(xf), B, hw, one, ptr(net.sums),
-         ptr(net.gout) if grad else None, st)
-    if grad:
-        net.backward(B)
-    return net.sums[:2 * B].view(B, 2)
-
-
-def train(net, pool, steps, batch=8, lr=2e-3, seed=0, log_every=0, log=print):
-    """Adam with cosine decay on random minibatches of the pool; returns the
-    per-log-step mean relative residuals."""
-    torch = _torch()
-    rng = np.random.default_rng(seed)
-    net.ensure(batch)
-    V = torch.empty((batch, pool.mats[0].factor, net.H, net.W), dtype=torch.float32, device=net.device)
-    hist = []
-    for it in range(steps):
-        idx = rng.choice(len(pool), size=batch, replace=False)
-        sums = batch_loss(net, pool, idx, V, grad=True)
-        net.adam(lr * 0.5 * (1 + math.cos(math.pi * it / steps)))
-        if log_every and (it % log_every == 0 or it == steps - 1):
-            r = float((sums[:, 0] / sums[:, 1]).mean())
-            hist.append((it, r))
-            log(f"step {it:5d}  relative residual {r:.4f}")
-    return hist
-
-
-def evaluate(net, pool, batch=8):
-    """Mean relative residual over the whole pool, corrected and uncorrected (= 1)."""
-    torch = _torch()
-    net.ensure(batch)
-    V = torch.empty((batch, pool.mats[0].factor, net.H, net.W), dtype=torch.float32, device=net.device)
-    num = den = 0.0
-    per = []
-    for i0 in range(0, len(pool), batch):
-        idx = list(range(i0, min(len(pool), i0 + batch)))
-        s = batch_loss(net, pool, idx, V).cpu().numpy()
-        per.extend(list(s[:, 0] / s[:, 1]))
-        num += s[:, 0].sum()
-        den += s[:, 1].sum()
-    return float(num / den), np.asarray(per)
-
-
 # ====================================================================== spectral corrector
 # One correction per probe, applied to the coarse path's exit spectrum (the
 # complex amplitude whose squared modulus is the CBED):
@@ -633,14 +364,16 @@ class SpectralMaterial:
         call("namsa_fft2d", ptr(out), B, self.Ny, self.Nx, 0, stream_handle())
         return out
 
-    def cbed(self, pos, S, factor, out):
-        """Plain-path CBEDs [B][Ny][Nx] after S slices of dz * factor (the fused driver)."""
+    def cbed(self, pos, S, factor, out, group=None):
+        """Plain-path CBEDs [B][Ny][Nx] after S slices of dz * factor (the
+        fused driver, `group` probes through the slices together; default all)."""
         torch = _torch()
         B = pos.shape[0]
+        G = min(B, int(group)) if group else B
         t, pm = self._slices(factor)
-        ws = torch.empty((B, self.Ny, self.Nx), dtype=torch.complex64, device=out.device)
+        ws = torch.empty((G, self.Ny, self.Nx), dtype=torch.complex64, device=out.device)
         call("namsa_multislice", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
-             np.float32(self.sim.kmax), ptr(t), int(S), ptr(pm), self.band[0], self.band[1], B, _lib.OUT_CBED,
+             np.float32(self.sim.kmax), ptr(t), int(S), ptr(pm), self.band[0], self.band[1], G, _lib.OUT_CBED,
              np.float32(1.0 / self.norm), ptr(ws), ptr(out), None, 0, None, None, stream_handle())
         return out
 
@@ -687,7 +420,7 @@ class SpectralCorrector:
         self.kmax, self.lam, self.dz_coarse, self.opts = float(kmax), float(lam), float(dz_coarse), int(opts)
         cin = 6 + (1 if opts & OPT_POTENTIAL else 0) + (1 if opts & OPT_AMPLITUDE else 0)
         _, _, H, W = self.window
-        self.net = FnoNet(H, W, cin=cin, C=C, L=L, m=m, Hq=Hq, seed=seed, device=device, derivs=False)
+        self.net = FnoNet(H, W, cin=cin, C=C, L=L, m=m, Hq=Hq, seed=seed, device=device)
         self.s = float(math.sqrt(self.Ny * self.Nx))
         self.phase_scale = float(math.pi * self.lam * self.dz_coarse / SPECTRAL_PHASE_DIV)
 
@@ -921,3 +654,33 @@ def make_spectral_pools(seeds, factors=(8,), probes=4, grid=256, pix=0.125, kv=2
                 f"snapshots {[u * unit for u in snaps]} A, {len(pool)} pairs")
         del sim
     return pool.finalize(), mats
+
+
+def run_spectral(corr, mat, pos, thickness, chunk=None, vhat=None):
+    """The learned coarse path: CBEDs [B][Ny][Nx] of probes at pos through
+    thickness / (factor dz) coarse slices, each probe's exit spectrum
+    corrected once by the network.  chunk: probes per network call (bounds
+    the activation memory); vhat: the potential spectrum over the thickness
+    (computed when None)."""
+    torch = _torch()
+    B = pos.shape[0]
+    f = corr.factor
+    S_f = int(round(thickness / mat.dz))
+    if S_f % f or S_f > mat.S_f:
+        raise NamsaError(f"thickness {thickness} is not a whole number of {f * mat.dz} A slices of the material")
+    dev = pos.device
+    shape = (mat.Ny, mat.Nx)
+    if vhat is None:
+        vhat = mat.potential_spectrum(S_f, torch.empty(shape, dtype=torch.complex64, device=dev))
+    chunk = max(1, min(int(chunk or B), B))
+    cb = torch.empty((B,) + shape, dtype=torch.float32, device=dev)
+    km = torch.full((chunk,), float(mat.sim.kmax), dtype=torch.float32, device=dev)
+    th = torch.full((chunk,), float(S_f * mat.dz), dtype=torch.float32, device=dev)
+    pc = torch.empty((chunk,) + shape, dtype=torch.complex64, device=dev)
+    p0 = torch.empty_like(pc)
+    for s in range(0, B, chunk):
+        b = min(chunk, B - s)
+        mat.exit_spectra(pos[s:s + b], S_f // f, f, pc[:b])
+        mat.vacuum_spectra(pos[s:s + b], S_f * mat.dz, p0[:b])
+        corr.correct(pc[:b], p0[:b], vhat.view((1,) + shape), None, km[:b], th[:b], cb[s:s + b])
+    return cb

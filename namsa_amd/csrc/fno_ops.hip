@@ -1,6 +1,7 @@
 // Pointwise kernels of the coarse-slice corrector network (its matrix
-// products run through namsa_gemm_bf16): fills, GELU backward, Adam, input
-// packing, the relative residual loss and the correction itself.
+// products run through namsa_gemm_bf16): fills, GELU backward, Adam, the
+// spectral modes' bookkeeping, and the spectral corrector's input packing,
+// CBED-intensity loss and correction.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,103 +38,6 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
     v[i] = vi;
     p[i] -= lr * ((mi / bc1) / (sqrtf(vi / bc2) + eps) + wd * p[i]);
   }
-}
-
-// u[0..1][n] = (re, im) psi_c * s; with derivs, u[2..7][n] = (re, im) of the
-// periodic 5-point Laplacian, the central y difference and the central x
-// difference of psi_c * s; then u[.. + j][n] = V_j * sv for j < nv.  n runs
-// over B*Ny*Nx; V is [B][nv][Ny*Nx], or [nv][Ny*Nx] shared by every sample
-// when v_shared.
-__global__ void pack_inputs_kernel(const float2* __restrict__ psi, const float* __restrict__ V, int nv,
-                                   int v_shared, int derivs, float* __restrict__ u, long long n, int Ny,
-                                   int Nx, float s, float sv) {
-  const long long hw = static_cast<long long>(Ny) * Nx;
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
-    const float2 c = psi[i];
-    u[i] = c.x * s;
-    u[n + i] = c.y * s;
-    const long long b = i / hw, px = i % hw;
-    int ch = 2;
-    if (derivs) {
-      const int y = static_cast<int>(px / Nx), x = static_cast<int>(px % Nx);
-      const float2* P = psi + b * hw;
-      const float2 up = P[static_cast<long long>(y == 0 ? Ny - 1 : y - 1) * Nx + x];
-      const float2 dn = P[static_cast<long long>(y == Ny - 1 ? 0 : y + 1) * Nx + x];
-      const float2 lf = P[static_cast<long long>(y) * Nx + (x == 0 ? Nx - 1 : x - 1)];
-      const float2 rt = P[static_cast<long long>(y) * Nx + (x == Nx - 1 ? 0 : x + 1)];
-      u[2 * n + i] = (up.x + dn.x + lf.x + rt.x - 4.0f * c.x) * s;
-      u[3 * n + i] = (up.y + dn.y + lf.y + rt.y - 4.0f * c.y) * s;
-      u[4 * n + i] = 0.5f * (dn.x - up.x) * s;
-      u[5 * n + i] = 0.5f * (dn.y - up.y) * s;
-      u[6 * n + i] = 0.5f * (rt.x - lf.x) * s;
-      u[7 * n + i] = 0.5f * (rt.y - lf.y) * s;
-      ch = 8;
-    }
-    const float* vb = V + (v_shared ? 0 : b * nv * hw) + px;
-    for (int j = 0; j < nv; ++j) u[(ch + j) * n + i] = vb[static_cast<long long>(j) * hw] * sv;
-  }
-}
-
-// out = |F|^2 * scale, times mask[i % hw] when a mask is given
-__global__ void intensity_kernel(const float2* __restrict__ F, long long n, long long hw, float scale,
-                                 const float* __restrict__ mask, float* __restrict__ out) {
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += 256LL * gridDim.x) {
-    const float2 f = F[i];
-    const float w = mask ? mask[i % hw] * scale : scale;
-    out[i] = (f.x * f.x + f.y * f.y) * w;
-  }
-}
-
-// per sample b: num = sum |out - (psi_f - psi_c) s|^2, den = sum |(psi_f - psi_c) s|^2
-__global__ void __launch_bounds__(1024)
-residual_sums_kernel(const float* __restrict__ out, const float2* __restrict__ pc,
-                     const float2* __restrict__ pf, int hw, long long n, float s,
-                     double* __restrict__ sums) {
-  __shared__ double red[2][16];
-  const long long b = blockIdx.x;
-  double num = 0.0, den = 0.0;
-  for (int i = threadIdx.x; i < hw; i += 1024) {
-    const long long j = b * hw + i;
-    const float dr = (pf[j].x - pc[j].x) * s, di = (pf[j].y - pc[j].y) * s;
-    const float rr = out[j] - dr, ri = out[n + j] - di;
-    num += double(rr) * rr + double(ri) * ri;
-    den += double(dr) * dr + double(di) * di;
-  }
-  num = wave_sum(num);
-  den = wave_sum(den);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = num;
-    red[1][threadIdx.x >> 6] = den;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0, c = 0.0;
-    for (int w = 0; w < 16; ++w) {
-      a += red[0][w];
-      c += red[1][w];
-    }
-    sums[2 * b] = a;
-    sums[2 * b + 1] = c;
-  }
-}
-
-// gout = 2 (out - target) / (den_b * B): gradient of mean_b num_b / den_b
-__global__ void residual_grad_kernel(const float* __restrict__ out, const float2* __restrict__ pc,
-                                     const float2* __restrict__ pf, int hw, long long n, float s,
-                                     const double* __restrict__ sums, int B, float* __restrict__ gout) {
-  for (long long j = blockIdx.x * 256LL + threadIdx.x; j < n; j += 256LL * gridDim.x) {
-    const long long b = j / hw;
-    const float w = float(2.0 / ((sums[2 * b + 1] + 1e-30) * B));
-    gout[j] = (out[j] - (pf[j].x - pc[j].x) * s) * w;
-    gout[n + j] = (out[n + j] - (pf[j].y - pc[j].y) * s) * w;
-  }
-}
-
-// psi = psi_c + out / s
-__global__ void apply_correction_kernel(const float* __restrict__ out, const float2* __restrict__ pc,
-                                        float2* __restrict__ psi, long long n, float inv_s) {
-  for (long long j = blockIdx.x * 256LL + threadIdx.x; j < n; j += 256LL * gridDim.x)
-    psi[j] = make_float2(pc[j].x + out[j] * inv_s, pc[j].y + out[n + j] * inv_s);
 }
 
 // P[cb][(r,p)][(t,q)] -> Xs[pq][(r,c)][b]; one thread per (pq, c, b)
@@ -364,40 +268,6 @@ int namsa_adam(float* p, const float* g, float* m, float* v, long long n, float 
   return check_hip(hipGetLastError(), "adam_kernel launch");
 }
 
-int namsa_corrector_pack(const void* psi_c, const float* V, int nv, int v_shared, int derivs, float* u, int B,
-                         int Ny, int Nx, float s, float sv, void* stream) {
-  if (!psi_c || (nv > 0 && !V) || nv < 0 || !u || B <= 0 || Ny <= 0 || Nx <= 0)
-    return fail(kErrArg, "namsa_corrector_pack: bad arguments");
-  const long long n = static_cast<long long>(B) * Ny * Nx;
-  hipLaunchKernelGGL(pack_inputs_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream),
-                     static_cast<const float2*>(psi_c), V, nv, v_shared, derivs ? 1 : 0, u, n, Ny, Nx, s, sv);
-  return check_hip(hipGetLastError(), "pack_inputs_kernel launch");
-}
-
-int namsa_intensity(const void* F, long long n, long long hw, float scale, const float* mask, float* out,
-                    void* stream) {
-  if (!F || !out || n <= 0 || hw <= 0) return fail(kErrArg, "namsa_intensity: bad arguments");
-  hipLaunchKernelGGL(intensity_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream),
-                     static_cast<const float2*>(F), n, hw, scale, mask, out);
-  return check_hip(hipGetLastError(), "intensity_kernel launch");
-}
-
-int namsa_corrector_loss(const float* out, const void* psi_c, const void* psi_f, int B, int hw, float s,
-                         double* sums, float* gout, void* stream) {
-  if (!out || !psi_c || !psi_f || !sums || B <= 0 || B > 65535 || hw <= 0)
-    return fail(kErrArg, "namsa_corrector_loss: bad arguments");
-  const long long n = static_cast<long long>(B) * hw;
-  const float2* pc = static_cast<const float2*>(psi_c);
-  const float2* pf = static_cast<const float2*>(psi_f);
-  hipLaunchKernelGGL(residual_sums_kernel, dim3(B), dim3(1024), 0, as_stream(stream), out, pc, pf, hw, n,
-                     s, sums);
-  if (int rc = check_hip(hipGetLastError(), "residual_sums_kernel launch")) return rc;
-  if (!gout) return kOk;
-  hipLaunchKernelGGL(residual_grad_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream), out, pc, pf,
-                     hw, n, s, sums, B, gout);
-  return check_hip(hipGetLastError(), "residual_grad_kernel launch");
-}
-
 int namsa_modes_combine(const float* P, float* Xs, int C, int B, int m, void* stream) {
   if (!P || !Xs || C <= 0 || B <= 0 || m <= 0) return fail(kErrArg, "namsa_modes_combine: bad arguments");
   const long long n = static_cast<long long>(m) * m * C * B;
@@ -462,16 +332,6 @@ int namsa_real_to_complex(const float* v, int B, int Ny, int Nx, float scale, in
   hipLaunchKernelGGL(real_to_complex_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream), v, n, Ny, Nx, scale,
                      checker ? 1 : 0, static_cast<float2*>(out));
   return check_hip(hipGetLastError(), "real_to_complex_kernel launch");
-}
-
-int namsa_corrector_apply(const float* out, const void* psi_c, void* psi, int B, int hw, float s,
-                          void* stream) {
-  if (!out || !psi_c || !psi || B <= 0 || hw <= 0 || s == 0.0f)
-    return fail(kErrArg, "namsa_corrector_apply: bad arguments");
-  const long long n = static_cast<long long>(B) * hw;
-  hipLaunchKernelGGL(apply_correction_kernel, dim3(ew_grid(n)), dim3(256), 0, as_stream(stream), out,
-                     static_cast<const float2*>(psi_c), static_cast<float2*>(psi), n, 1.0f / s);
-  return check_hip(hipGetLastError(), "apply_correction_kernel launch");
 }
 
 }  // extern "C"

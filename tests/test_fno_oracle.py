@@ -153,3 +153,21 @@ def test_spectral_apply_and_inputs_closed_forms():
     assert u[2, 0, 3, 3] == pytest.approx(2.0 * pc[2, r0 + 3, c0 + 3].real)
     assert u[2, 3, 3, 3] == pytest.approx(2.0 * p0[2, r0 + 3, c0 + 3].imag)
     assert u[0, 7, 5, 6] == pytest.approx(2.0 * abs(pc[0, r0 + 5, c0 + 6]))
+
+
+def test_committed_corrector_has_the_specified_architecture():
+    """namsa_amd/data/corrector_c4_f8.npz: 4 Fourier layers, 32 channels, 16 x 16
+    modes, 7 inputs (coarse and vacuum spectra, phase, thickness, potential
+    spectrum), factor 8 at 256^2; the weight count matches those shapes."""
+    import json
+
+    d = np.load("namsa_amd/data/corrector_c4_f8.npz", allow_pickle=False)
+    cfg = json.loads(str(d["config"]))
+    assert (cfg["L"], cfg["C"], cfg["m"], cfg["Hq"]) == (4, 32, 16, 64)
+    assert cfg["factor"] == 8 and cfg["dz_coarse"] == 8.0 and cfg["opts"] == 1
+    assert (cfg["Ny"], cfg["Nx"]) == (256, 256) and cfg["window"] == [43, 43, 170, 170]
+    C, L, m, Hq, cin = 32, 4, 16, 64, 7
+    sizes = [C * cin, C, L * m * m * 4 * C * C, L * C * C, L * C, Hq * C, Hq, 2 * Hq, 2]
+    assert d["theta_bf16"].size == sum((n + 63) // 64 * 64 for n in sizes)
+    w = (d["theta_bf16"].astype(np.uint32) << 16).view(np.float32)
+    assert np.isfinite(w).all() and np.abs(w).max() < 1e3

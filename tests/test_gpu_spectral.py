@@ -188,3 +188,38 @@ def test_spectral_training_lowers_the_cbed_error(material):
     after, _ = spectral_evaluate(corr, pool, batch=8)
     _say(f"  training pairs: coarse mean rel-L2 {coarse.mean():.4f}, after 400 steps {after.mean():.4f}")
     assert after.mean() < 0.97 * coarse.mean()    # measured: 0.0212 -> 0.0196 after 300 steps
+
+
+def test_trained_corrector_gains_on_held_out_structures(gpu):
+    """What the trained network gains over the method without it: the
+    committed factor-8 network (trained on seeds from 100000, scored there on
+    seeds from 900000) on 12 random crystals from a third seed range, 4 probes
+    each, at their full thickness.  The learned CBEDs must be closer to the
+    fine (dz = 1 A) path than the plain coarse (dz = 8 A) ones on average and
+    for most probes.  Measured on 1916 held-out pairs: mean rel-L2 0.0777
+    learned against 0.0929 coarse, 93% of probes improved."""
+    import torch
+
+    from namsa_amd.corrector import SpectralCorrector, SpectralMaterial, random_crystal_sim, run_spectral
+
+    corr = SpectralCorrector.load("namsa_amd/data/corrector_c4_f8.npz")
+    e_c, e_l = [], []
+    for seed in range(970_000, 970_012):
+        sim, rng, _ = random_crystal_sim(seed)
+        mat = SpectralMaterial(sim, factors=(8,))
+        S = mat.S_f // 8 * 8
+        L = float(min(sim.dims[:2]))
+        pos = torch.from_numpy(rng.uniform(-L / 4, L / 4, (4, 2)).astype(np.float32)).to(sim.device)
+        fine = mat.cbed(pos, S, None, torch.empty((4, 256, 256), device=sim.device)).cpu().numpy()
+        coarse = mat.cbed(pos, S // 8, 8, torch.empty((4, 256, 256), device=sim.device)).cpu().numpy()
+        learned = run_spectral(corr, mat, pos, float(S)).cpu().numpy()
+        for b in range(4):
+            n = np.linalg.norm(fine[b])
+            e_c.append(np.linalg.norm(coarse[b] - fine[b]) / n)
+            e_l.append(np.linalg.norm(learned[b] - fine[b]) / n)
+    e_c, e_l = np.array(e_c), np.array(e_l)
+    frac = float((e_l < e_c).mean())
+    _say(f"  48 held-out probes: CBED rel-L2 vs fine, plain coarse mean {e_c.mean():.4f} median {np.median(e_c):.4f}; "
+         f"learned mean {e_l.mean():.4f} median {np.median(e_l):.4f}; improved {frac:.0%}")
+    assert e_l.mean() < 0.95 * e_c.mean()
+    assert frac >= 0.7
