@@ -68,3 +68,32 @@ def test_batch_material_matches_oracle(gpu):
         err = max(rel_l2(m["cbed"][b].cpu().numpy(), cb_ref[b]) for b in range(m["B"]))
         print(f"material {i} N={m['N']} S={m['S']} B={m['B']} CBED relL2 {err:.2e}", flush=True)
         assert err < 1e-4
+
+
+def test_batch_large_grids_match_oracle(gpu):
+    """512^2, 768^2 and 1024^2 materials inside one captured batch (with a
+    256^2 one), replayed; the first probe of each large material against the
+    FP64 oracle at the 1e-4 CBED bar."""
+    import torch
+
+    specs = []
+    for k, N in enumerate((256, 512, 768, 1024)):
+        specs += structures.multi_material_specs(seed=70 + k, count=1, grids=(N,), thickness=(6.0, 12.0),
+                                                 probes=(2, 3), pix=(0.1, 0.12))
+    batch = MaterialBatch(specs, streams=2)
+    assert sorted(m["N"] for m in batch.mats) == [256, 512, 768, 1024]
+    batch.capture()
+    batch.replay()
+    batch.replay()
+    torch.cuda.synchronize()
+    for i, m in enumerate(batch.mats):
+        if m["N"] < 512:
+            continue
+        sim = m["sim"]
+        t_ref, _ = oracle_slices(sim, specs[i]["dz"])
+        psi_k, _ = oracle_probe(sim)
+        pm = oracle_pm(sim, specs[i]["dz"], 1 / 3)
+        cb_ref = O.cbed(psi_k, sim.probe_positions[:1].astype(np.float32), np.float32(sim.kmax), t_ref, pm)
+        err = rel_l2(m["cbed"][0].cpu().numpy(), cb_ref[0])
+        print(f"material {i} N={m['N']} S={m['S']} B={m['B']} CBED relL2 {err:.2e}", flush=True)
+        assert err < 1e-4
