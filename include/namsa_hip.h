@@ -326,6 +326,27 @@ int namsa_adam(float* p, const float* g, float* m, float* v, long long n, float 
 int namsa_modes_combine(const float* P, float* Xs, int C, int B, int m, void* stream);
 int namsa_modes_expand(const float* Ys, float* E, int C, int B, int m, void* stream);
 /*
+ * Channel-last BF16 inference of the corrector network (32 channels, 2m = 32
+ * row-DFT columns, head width 64), no counterpart in the reference.
+ * Activations x: bf16 [B*H][W][32]; Z / Zp: float [(c*B + b)*H + y][32] (the
+ * row DFT and row-inverse operands the strided GEMM path's mode-space
+ * products read and write); fx: bf16 [Wp][32] = Fx[x][q] (the truncated row
+ * DFT, rows >= W zero), fxT: bf16 [32][Wp] its transpose (columns >= W zero),
+ * Wp a multiple of 32 >= W.
+ *   namsa_fno_lift:  x0 = Wl u + bl (u float [cin][B*H*W], cin <= 8, Wl
+ *                    [32][cin]) and Z = row DFT of x0
+ *   namsa_fno_layer: x_out = gelu(alpha Fx Zp + Wt x_in + bias) and Z_next =
+ *                    row DFT of x_out; with out != NULL (the last layer) no
+ *                    GELU and out[t][p] = Q2[t] . gelu(Q1 x + bq1) + bq2[t]
+ *                    (float [2][B*H*W]) instead of x_out and Z_next.
+ */
+int namsa_fno_lift(const float* u, int cin, const float* Wl, const float* bl, const void* fxT, int B, int H, int W,
+                   int Wp, void* x0, float* Z, void* stream);
+int namsa_fno_layer(const void* x_in, const float* Zp, const void* fx, const void* fxT, const float* Wt,
+                    const float* bias, float alpha, int B, int H, int W, int Wp, void* x_out, float* Z_next,
+                    const float* Q1, const float* bq1, const float* Q2, const float* bq2, float* out, void* stream);
+
+/*
  * Spectral (one-shot) corrector: works on the centred exit spectrum Psi(k) of
  * the coarse path, |Psi|^2 being the CBED, inside the window
  * [r0, r0+H) x [c0, c0+W) of the Ny x Nx plane that bounds the bandwidth mask.

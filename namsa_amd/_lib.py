@@ -75,6 +75,9 @@ SIGNATURES = {
     "namsa_spectral_loss": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp, _vp]),
     "namsa_spectral_apply": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _f, _vp, _vp]),
     "namsa_real_to_complex": (_i, [_vp, _i, _i, _i, _f, _i, _vp, _vp]),
+    "namsa_fno_lift": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "namsa_fno_layer": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp]),
     "namsa_modes_combine": (_i, [_vp, _vp, _i, _i, _i, _vp]),
     "namsa_modes_expand": (_i, [_vp, _vp, _i, _i, _i, _vp]),
 }

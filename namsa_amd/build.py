@@ -22,7 +22,7 @@ INCLUDE = ROOT / "include"
 OBJDIR = PKG / "_build"
 LIB = PKG / "libnamsa_hip.so"
 SOURCES = ["abi_common.hip", "multislice.hip", "slices.hip", "passes.hip", "gemm_mfma.hip", "fno_ops.hip",
-           "records.hip"]
+           "fno_fused.hip", "records.hip"]
 PASS_SIZES = [128, 192, 256, 384, 512, 768, 1024, 2048]
 ARCH = os.environ.get("NAMSA_OFFLOAD_ARCH", "gfx950")
 

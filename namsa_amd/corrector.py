@@ -78,7 +78,10 @@ class FnoNet:
         fx = np.concatenate([np.cos(tx), -np.sin(tx)], 1)            # [x][(t,q)]
         self.Fy = torch.from_numpy(fy.astype(np.float32)).to(self.device)
         self.Fx = torch.from_numpy(fx.astype(np.float32)).to(self.device)
-        self.B = 0
+        self.B = 0                  # samples the training buffers hold
+        self.B_common = 0           # ... the mode-space and input / output buffers
+        self.B_fused = 0            # ... the channel-last inference buffers
+        self.fx_bf = None
         self.st = stream_handle()
 
     # ------------------------------------------------------------------ parameters
@@ -131,19 +134,19 @@ class FnoNet:
         return _addr(self.grad, o)
 
     # ------------------------------------------------------------------ buffers
-    def ensure(self, B):
-        """Activation buffers for batches of up to B samples."""
-        if B <= self.B:
+    def _ensure_common(self, B):
+        """Mode-space buffers and the network's input / output for B samples (both forward paths)."""
+        if B <= self.B_common:
             return
         torch = _torch()
         if self.C * B > 65535:
             raise NamsaError("batch too large for the batched mode transforms")
-        H, W, C, L, m, Hq = self.H, self.W, self.C, self.L, self.m, self.Hq
+        H, W, C, L, m = self.H, self.W, self.C, self.L, self.m
         n = B * H * W
         f = dict(dtype=torch.float32, device=self.device)
         self.u = torch.empty((self.cin, n), **f)
-        self.xs = [torch.empty((C, n), **f) for _ in range(L + 1)]
-        self.pre = [torch.empty((C, n), **f) for _ in range(L - 1)]
+        self.out = torch.empty((2, n), **f)
+        self.gout = torch.empty((2, n), **f)
         self.Xs = torch.empty((L, m * m, 2 * C, B), **f)
         self.Z = torch.empty((C * B * H, 2 * m), **f)
         self.Pm = torch.empty((C * B, 2 * m, 2 * m), **f)
@@ -151,17 +154,50 @@ class FnoNet:
         self.gYs = torch.empty((m * m, 2 * C, B), **f)
         self.E = torch.empty((C * B, 2 * m, 2 * m), **f)
         self.Zp = torch.empty((C * B * H, 2 * m), **f)
+        self.sums = torch.empty(2 * B, dtype=torch.float64, device=self.device)
+        self.B_common = B
+
+    def ensure(self, B):
+        """Buffers of the training forward and backward (FP32 activations) for up to B samples."""
+        self._ensure_common(B)
+        if B <= self.B:
+            return
+        torch = _torch()
+        H, W, C, L, Hq = self.H, self.W, self.C, self.L, self.Hq
+        n = B * H * W
+        f = dict(dtype=torch.float32, device=self.device)
+        self.xs = [torch.empty((C, n), **f) for _ in range(L + 1)]
+        self.pre = [torch.empty((C, n), **f) for _ in range(L - 1)]
         self.hpre = torch.empty((Hq, n), **f)
         self.h = torch.empty((Hq, n), **f)
-        self.out = torch.empty((2, n), **f)
-        self.gout = torch.empty((2, n), **f)
         self.ga = torch.empty((C, n), **f)
         self.gb = torch.empty((C, n), **f)
         self.gh = torch.empty((Hq, n), **f)
         self.ones = torch.empty(n, **f)
         call("namsa_fill", ptr(self.ones), n, np.float32(1.0), self.st)
-        self.sums = torch.empty(2 * B, dtype=torch.float64, device=self.device)
         self.B = B
+
+    def fused_ok(self):
+        """The channel-last BF16 kernels are compiled for 32 channels, 16 modes per axis, head width 64."""
+        return (self.C, self.m, self.Hq) == (32, 16, 64) and self.cin <= 8
+
+    def ensure_fused(self, B):
+        """Buffers of the channel-last BF16 inference forward for up to B samples."""
+        self._ensure_common(B)
+        if B <= self.B_fused:
+            return
+        torch = _torch()
+        n = B * self.H * self.W
+        self.xa = torch.empty((n, self.C), dtype=torch.bfloat16, device=self.device)
+        self.xb = torch.empty((n, self.C), dtype=torch.bfloat16, device=self.device)
+        if self.fx_bf is None:
+            Wp = (self.W + 31) // 32 * 32
+            fx = torch.zeros((Wp, 2 * self.m), dtype=torch.float32, device=self.device)
+            fx[:self.W] = self.Fx
+            self.Wp = Wp
+            self.fx_bf = fx.to(torch.bfloat16).contiguous()
+            self.fxT_bf = fx.t().contiguous().to(torch.bfloat16)
+        self.B_fused = B
 
     # ------------------------------------------------------------------ GEMM
     def _gemm(self, A, sA, Bm, sB, Cm, sC, M, N, K, batch=1, split=1, alpha=1.0, beta=0.0, bias=None,
@@ -216,6 +252,43 @@ class FnoNet:
                    bias=self._p("bq1"), bias_mode=1, gelu=True, P=ptr(self.hpre))
         self._gemm(self._p("Q2"), (0, Hq, 1), ptr(self.h), (0, n, 1), ptr(self.out), (0, n, 1), 2, n, Hq,
                    bias=self._p("bq2"), bias_mode=1)
+        return self.out
+
+    def forward_fused(self, B):
+        """out[2][B*H*W] from u[cin][B*H*W] through the channel-last BF16
+        kernels (inference only: nothing is kept for a backward pass).  The
+        same network as forward(); activations are rounded to bf16 between
+        layers instead of held in FP32."""
+        if not self.fused_ok():
+            raise NamsaError("the fused forward needs C=32, m=16, Hq=64 and at most 8 inputs")
+        self.ensure_fused(B)
+        H, W, C, L, m = self.H, self.W, self.C, self.L, self.m
+        C2 = 2 * C
+        m2, CB = 2 * m, C * B
+        st = self.st
+        call("namsa_fno_lift", ptr(self.u), self.cin, self._p("Wl"), self._p("bl"), ptr(self.fxT_bf), B, H, W,
+             self.Wp, ptr(self.xa), ptr(self.Z), st)
+        x_in, x_out = self.xa, self.xb
+        for l in range(L):
+            Xs = _addr(self.Xs, l * m * m * C2 * B)
+            self._gemm(_addr(self.Fy), (0, H, 1), _addr(self.Z), (H * m2, m2, 1), _addr(self.Pm),
+                       (m2 * m2, m2, 1), m2, m2, H, batch=CB)
+            call("namsa_modes_combine", ptr(self.Pm), Xs, C, B, m, st)
+            self._gemm(self._p("G", l), (C2 * C2, C2, 1), Xs, (C2 * B, B, 1), ptr(self.Ys), (C2 * B, B, 1),
+                       C2, B, C2, batch=m * m)
+            call("namsa_modes_expand", ptr(self.Ys), ptr(self.E), C, B, m, st)
+            self._gemm(_addr(self.Fy), (0, 1, H), _addr(self.E), (m2 * m2, m2, 1), _addr(self.Zp),
+                       (H * m2, m2, 1), H, m2, m2, batch=CB)
+            alpha = np.float32(1.0 / (H * W))
+            if l < L - 1:
+                call("namsa_fno_layer", ptr(x_in), ptr(self.Zp), ptr(self.fx_bf), ptr(self.fxT_bf),
+                     self._p("W", l), self._p("b", l), alpha, B, H, W, self.Wp, ptr(x_out), ptr(self.Z), None, None,
+                     None, None, None, st)
+                x_in, x_out = x_out, x_in
+            else:
+                call("namsa_fno_layer", ptr(x_in), ptr(self.Zp), ptr(self.fx_bf), ptr(self.fxT_bf),
+                     self._p("W", l), self._p("b", l), alpha, B, H, W, self.Wp, None, None, self._p("Q1"),
+                     self._p("bq1"), self._p("Q2"), self._p("bq2"), ptr(self.out), st)
         return self.out
 
     def backward(self, B):
@@ -423,6 +496,7 @@ class SpectralCorrector:
         self.net = FnoNet(H, W, cin=cin, C=C, L=L, m=m, Hq=Hq, seed=seed, device=device)
         self.s = float(math.sqrt(self.Ny * self.Nx))
         self.phase_scale = float(math.pi * self.lam * self.dz_coarse / SPECTRAL_PHASE_DIV)
+        self.fused = True           # inference through the channel-last BF16 kernels where they apply
 
     def config(self):
         n = self.net
@@ -453,7 +527,7 @@ class SpectralCorrector:
 
     def pack(self, psi_c, psi_0, vhat, vidx, kmax, thick, B):
         r0, c0, H, W = self.window
-        self.net.ensure(B)
+        self.net._ensure_common(B)
         nv = int(vhat.shape[0]) if vhat is not None and vhat.dim() == 3 else 1
         call("namsa_spectral_pack", ptr(psi_c), ptr(psi_0), ptr(vhat), ptr(vidx), nv, B, self.Ny, self.Nx, r0, c0, H,
              W, ptr(kmax), np.float32(self.s), np.float32(self.phase_scale), ptr(thick),
@@ -476,7 +550,10 @@ class SpectralCorrector:
         """CBEDs [B][Ny][Nx] of corrected coarse exit spectra (kmax, thick: device [B] float32)."""
         B = psi_c.shape[0]
         self.pack(psi_c, psi_0, vhat, vidx, kmax, thick, B)
-        self.net.forward(B)
+        if self.fused and self.net.fused_ok():
+            self.net.forward_fused(B)
+        else:
+            self.net.forward(B)
         return self.apply(psi_c, B, cbed)
 
 

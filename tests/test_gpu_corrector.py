@@ -61,3 +61,37 @@ def test_network_matches_oracle(gpu, cfg):
     errs = {k: _rel(g[k], gref[k]) for k in net.shapes}
     _say("  grads relL2 " + " ".join(f"{k}={v:.1e}" for k, v in errs.items()))
     assert max(errs.values()) < 5e-2
+
+
+@pytest.mark.parametrize("H,W,B", [
+    (37, 70, 3),      # 111 rows: the last workgroup has a masked wavefront; W = 70 ends in a partial tile
+    (64, 64, 2),
+    (170, 170, 2),    # the config-4 window
+])
+def test_fused_forward_matches_oracle(gpu, H, W, B):
+    """Channel-last BF16 inference forward against the FP64 oracle and the
+    FP32-activation forward of the same weights.  Tolerance 3e-2: BF16
+    operands as in the strided path, plus activations rounded to bf16 between
+    layers."""
+    import torch
+    _say(f"fused forward vs oracle H={H} W={W} B={B}")
+    rng = np.random.default_rng(H + W)
+    net = K.FnoNet(H, W, cin=7, C=32, L=4, m=16, Hq=64, seed=5)
+    p = net.params_numpy()
+    for k in ("bl", "b", "bq1", "bq2"):
+        p[k] = rng.standard_normal(p[k].shape) * 0.2
+    p["Q2"] = rng.standard_normal(p["Q2"].shape) / np.sqrt(64)
+    net.load(p)
+    p = net.params_numpy()
+    u = rng.standard_normal((B, 7, H, W))
+    n = B * H * W
+    net.ensure(B)
+    net.u[:, :n].copy_(torch.from_numpy(u.transpose(1, 0, 2, 3).reshape(7, -1).astype(np.float32)))
+    fused = net.forward_fused(B)[:, :n].cpu().numpy().reshape(2, B, H, W).transpose(1, 0, 2, 3)
+    fp32 = net.forward(B)[:, :n].cpu().numpy().reshape(2, B, H, W).transpose(1, 0, 2, 3)
+    ref, _ = F.forward(p, u)
+    e_ref, e_fp32 = _rel(fused, ref), _rel(fused, fp32)
+    _say(f"  fused vs FP64 oracle {e_ref:.2e}, vs FP32-activation forward {e_fp32:.2e}, FP32 vs oracle "
+         f"{_rel(fp32, ref):.2e}")
+    assert np.isfinite(fused).all()
+    assert e_ref < 3e-2 and e_fp32 < 3e-2
