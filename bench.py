@@ -547,6 +547,28 @@ def bench_config3(args):
         e[1] += m["B"] * m["S"]
     log(f"config3 roofline: {alg / 1e9:.1f} GB algorithmic per replay, {achieved:.0f} GB/s "
         f"({achieved / HBM_PEAK_GBS:.2f} of {HBM_PEAK_GBS:.0f})")
+    # A/B against launching each material alone: per grid, the time its materials take one at a
+    # time (HIP events) and the rate on their algorithmic bytes
+    st = torch.cuda.current_stream()
+    alone = {}
+    for i, m in enumerate(batch.mats):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        batch._launch(i, st)
+        e0.record(st)
+        batch._launch(i, st)
+        e1.record(st)
+        torch.cuda.synchronize()
+        b = m["S"] * (32 * m["N"] ** 2 * m["B"] + 8 * m["N"] ** 2 * -(-m["B"] // m["group"]))
+        a = alone.setdefault(m["N"], [0.0, 0.0, 0])
+        a[0] += e0.elapsed_time(e1) * 1e-3
+        a[1] += b
+        a[2] += m["B"] * m["S"]
+    per_grid = {str(N): {"seconds_alone": v[0], "GBs_alone": v[1] / v[0] / 1e9, "probe_slices": v[2]}
+                for N, v in sorted(alone.items())}
+    t_alone = sum(v[0] for v in alone.values())
+    log("config3 per grid, each material alone: " + ", ".join(
+        f"{N}: {v['seconds_alone'] * 1e3:.0f} ms at {v['GBs_alone']:.0f} GB/s" for N, v in per_grid.items()))
+    log(f"config3: materials alone sum to {t_alone * 1e3:.0f} ms; one graph replay {times['graph'] * 1e3:.0f} ms")
     out = {
         "metric": METRIC, "value": ps / times["graph"], "unit": "probe-positions·slices/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": times["graph"] * 1e3,
@@ -559,7 +581,8 @@ def bench_config3(args):
                    "materials": len(batch.mats), "probe_slices": ps, "streams": args.streams,
                    "sequential_probe_slices_per_s": ps / times["sequential"],
                    "multistream_probe_slices_per_s": ps / times["streams"],
-                   "materials_and_probe_slices_per_grid": {str(k): v for k, v in sorted(per_size.items())}},
+                   "materials_and_probe_slices_per_grid": {str(k): v for k, v in sorted(per_size.items())},
+                   "each_material_alone_per_grid": per_grid, "each_material_alone_total_s": t_alone},
         "roofline": {"bound": "hbm", "kernel": "whole batch (row and column passes of every material)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_replay": alg,
@@ -593,6 +616,7 @@ def bench_config4(args):
         raise SystemExit("no corrector weights under namsa_amd/data; train them with python -m namsa_amd.train_corrector")
     factors = tuple(sorted(nets))
     errs = {f: {"coarse": [], "learned": []} for f in factors}
+    above = {f: 0 for f in factors}          # probes the default fallback threshold would recompute
     t0 = time.perf_counter()
     for seed in range(950_000, 950_000 + args.c4_structures):
         sim, rng, _ = K.random_crystal_sim(seed)
@@ -603,7 +627,9 @@ def bench_config4(args):
         fine = mat.cbed(pos, S, None, torch.empty((4, 256, 256), device=sim.device)).cpu().numpy()
         for f in factors:
             coarse = mat.cbed(pos, S // f, f, torch.empty((4, 256, 256), device=sim.device)).cpu().numpy()
-            learned = K.run_spectral(nets[f], mat, pos, float(S)).cpu().numpy()
+            st = {}
+            learned = K.run_spectral(nets[f], mat, pos, float(S), tau=1e30, stats=st).cpu().numpy()
+            above[f] += int((st["estimate"] > K.DEFAULT_TAU).sum())
             for b in range(4):
                 n = np.linalg.norm(fine[b])
                 errs[f]["coarse"].append(float(np.linalg.norm(coarse[b] - fine[b]) / n))
@@ -635,26 +661,18 @@ def bench_config4(args):
         for f in factors:
             vh = mat.potential_spectrum(T, torch.empty((256, 256), dtype=torch.complex64, device=sim.device))
             row[f"coarse_f{f}"] = rate(lambda: mat.cbed(pos, T // f, f, out), B)
-            row[f"learned_f{f}"] = rate(lambda: K.run_spectral(nets[f], mat, pos, float(T), vhat=vh), B)
+            row[f"learned_f{f}"] = rate(lambda: K.run_spectral(nets[f], mat, pos, float(T), vhat=vh, tau=None), B)
         curve[f"{T}A"] = row
         log(f"config4 throughput at {T} A (probe-positions/s): " + ", ".join(f"{k} {v:.0f}" for k, v in row.items()))
-    # the corrector's forward pass on the same batch, timed with events on its stream
+    # the corrector's forward pass (channel-last BF16 kernels) on the same batch, timed with events on its stream
     f = max(factors)
     net = nets[f].net
-    net.ensure(B)
-    net.forward(B)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(5):
-        net.forward(B)
-    ev1.record()
-    torch.cuda.synchronize()
-    t_fwd = ev0.elapsed_time(ev1) / 5 * 1e-3
-    tflops = net.flops(B) / t_fwd / 1e12
+    t_fwd, tflops, gbs = corrector_forward_rate(net, B)
     summary = {f: {"coarse_mean": float(np.mean(errs[f]["coarse"])), "learned_mean": float(np.mean(errs[f]["learned"])),
                    "coarse_median": float(np.median(errs[f]["coarse"])),
                    "learned_median": float(np.median(errs[f]["learned"])),
-                   "fraction_improved": float(np.mean(np.array(errs[f]["learned"]) < np.array(errs[f]["coarse"])))}
+                   "fraction_improved": float(np.mean(np.array(errs[f]["learned"]) < np.array(errs[f]["coarse"]))),
+                   "probes_above_default_fallback_tau": above[f]}
                for f in factors}
     for f in factors:
         log(f"config4 factor {f}: CBED rel-L2 vs fine, coarse mean {summary[f]['coarse_mean']:.4f} -> learned "
@@ -675,12 +693,38 @@ def bench_config4(args):
                    "cbed_relL2_vs_fine": {str(k): v for k, v in summary.items()},
                    "held_out_structures": args.c4_structures, "probes_per_structure": 4,
                    "throughput_probe_positions_per_s": curve},
-        "roofline": {"bound": "mfma", "kernel": "corrector forward (FnoNet, BF16 MFMA GEMMs)", "achieved": tflops,
-                     "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "traffic": None,
-                     "forward_ms_per_batch": t_fwd * 1e3, "batch": B, "flops_per_batch": net.flops(B)},
+        "roofline": corrector_roofline(net, B, t_fwd, tflops, gbs),
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+def corrector_forward_rate(net, B, reps=5):
+    """Seconds per forward_fused pass on B samples (HIP events on the corrector's
+    stream), its TFLOP/s and its algorithmic GB/s."""
+    import torch
+
+    net.ensure_fused(B)
+    net.forward_fused(B)
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        net.forward_fused(B)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t = ev0.elapsed_time(ev1) / reps * 1e-3
+    return t, net.flops(B) / t / 1e12, net.fused_bytes(B) / t / 1e9
+
+
+def corrector_roofline(net, B, t, tflops, gbs):
+    """The corrector forward is HBM-bound (about 30 kflop per pixel against
+    about 0.6 KB of bf16 activation traffic, below the 2.5 PFLOP/s / 8 TB/s
+    ridge): the HBM rate on its algorithmic bytes, with the MFMA rate beside it."""
+    return {"bound": "hbm", "kernel": "corrector forward (channel-last BF16 MFMA kernels + mode-space GEMMs)",
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_pass": net.fused_bytes(B), "forward_ms": t * 1e3, "batch": B,
+            "mfma_TFLOPs": tflops, "mfma_frac": tflops / 2500.0, "flops_per_pass": net.flops(B)}
 
 
 def bench_config5(args):
@@ -822,12 +866,12 @@ def bench_config5_learned(args):
         torch.cuda.synchronize()
         return time.perf_counter() - t1, r
 
-    K.run_spectral(corr, mat, pos[:chunk], T, chunk=chunk, vhat=vh)      # warm-up
+    K.run_spectral(corr, mat, pos[:chunk], T, chunk=chunk, vhat=vh, tau=None)      # warm-up
     t_fine, _ = timed(lambda: mat.cbed(pos, S_f, None, out, group=G))
     fine = out.cpu().numpy()
     t_coarse, _ = timed(lambda: mat.cbed(pos, S_f // 8, 8, out, group=G))
     coarse = out.cpu().numpy()
-    t_learned, cb = timed(lambda: K.run_spectral(corr, mat, pos, T, chunk=chunk, vhat=vh))
+    t_learned, cb = timed(lambda: K.run_spectral(corr, mat, pos, T, chunk=chunk, vhat=vh, tau=None))
     learned = cb.cpu().numpy()
     del cb
     e_c = np.array([np.linalg.norm(coarse[b] - fine[b]) / np.linalg.norm(fine[b]) for b in range(P)])
@@ -838,16 +882,7 @@ def bench_config5_learned(args):
         f"{e_c.mean():.4f} median {np.median(e_c):.4f}, learned mean {e_l.mean():.4f} median {np.median(e_l):.4f}")
     # the corrector forward for one chunk, timed with events on its stream
     net = corr.net
-    net.ensure(chunk)
-    net.forward(chunk)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(3):
-        net.forward(chunk)
-    ev1.record()
-    torch.cuda.synchronize()
-    t_fwd = ev0.elapsed_time(ev1) / 3 * 1e-3
-    tflops = net.flops(chunk) / t_fwd / 1e12
+    t_fwd, tflops, gbs = corrector_forward_rate(net, chunk, reps=3)
     res = {
         "metric": METRIC, "value": rates["learned"], "unit": "probe-positions/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": t_learned * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -863,9 +898,7 @@ def bench_config5_learned(args):
                                           "fraction_improved": float((e_l < e_c).mean())},
                    "fine_vs_fp64_oracle": "3.8e-5 (tests/full_size_config5.py, two positions)",
                    "chunk": chunk, "group": G},
-        "roofline": {"bound": "mfma", "kernel": "corrector forward (FnoNet, BF16 MFMA GEMMs)", "achieved": tflops,
-                     "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "traffic": None,
-                     "forward_ms_per_chunk": t_fwd * 1e3},
+        "roofline": corrector_roofline(net, chunk, t_fwd, tflops, gbs),
         "cpu_baseline": None,
     }
     print(json.dumps(res), flush=True)

@@ -82,7 +82,7 @@ class FnoNet:
         self.B_common = 0           # ... the mode-space and input / output buffers
         self.B_fused = 0            # ... the channel-last inference buffers
         self.fx_bf = None
-        self.st = stream_handle()
+        self._graphs = {}           # B -> captured forward_fused
 
     # ------------------------------------------------------------------ parameters
     def init_params(self, seed=0):
@@ -133,11 +133,17 @@ class FnoNet:
             o += idx * int(np.prod(self.shapes[name][1:]))
         return _addr(self.grad, o)
 
+    @property
+    def st(self):
+        """The current stream (taken at each call, so launches follow torch's stream and graph capture)."""
+        return stream_handle()
+
     # ------------------------------------------------------------------ buffers
     def _ensure_common(self, B):
         """Mode-space buffers and the network's input / output for B samples (both forward paths)."""
         if B <= self.B_common:
             return
+        self._graphs.clear()                      # captured graphs hold the addresses about to be replaced
         torch = _torch()
         if self.C * B > 65535:
             raise NamsaError("batch too large for the batched mode transforms")
@@ -186,6 +192,7 @@ class FnoNet:
         self._ensure_common(B)
         if B <= self.B_fused:
             return
+        self._graphs.clear()
         torch = _torch()
         n = B * self.H * self.W
         self.xa = torch.empty((n, self.C), dtype=torch.bfloat16, device=self.device)
@@ -290,6 +297,39 @@ class FnoNet:
                      self._p("W", l), self._p("b", l), alpha, B, H, W, self.Wp, None, None, self._p("Q1"),
                      self._p("bq1"), self._p("Q2"), self._p("bq2"), ptr(self.out), st)
         return self.out
+
+    def forward_graph(self, B):
+        """forward_fused(B) replayed from a HIP graph captured on first use for
+        this B (about 26 launches per pass: issued one by one from the host
+        they cost more time than the kernels take at 256^2).  The graph reads
+        u and writes out in place; the weights are read at replay time, so
+        loading new parameters needs no recapture."""
+        torch = _torch()
+        g = self._graphs.get(B)
+        if g is None:
+            self.forward_fused(B)                 # allocations and one-time tables outside the capture
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.forward_fused(B)
+            self._graphs[B] = g
+        g.replay()
+        return self.out
+
+    def fused_bytes(self, B):
+        """Algorithmic HBM bytes of one forward_fused pass on B samples: the
+        inputs read (4 cin bytes per pixel), the bf16 activation written by the
+        lift, read and written by each of the L - 1 GELU layers and read by the
+        last one (64 bytes per pixel each way), the two output channels (8
+        bytes), and per image row the 4 KB row-DFT operand each layer's row
+        kernels write or read (Z, Zp) plus the mode-space products' read of Z
+        and write of Zp."""
+        n = B * self.H * self.W
+        rows = B * self.H
+        act = 2 * self.C
+        per_pixel = 4 * self.cin + act + 2 * act * (self.L - 1) + act + 8
+        zrow = self.C * 2 * self.m * 4
+        return n * per_pixel + rows * zrow * (2 * self.L + 2 * self.L)
 
     def backward(self, B):
         """grad = d(sum out * gout)/d theta for the last forward of B samples."""
@@ -551,7 +591,7 @@ class SpectralCorrector:
         B = psi_c.shape[0]
         self.pack(psi_c, psi_0, vhat, vidx, kmax, thick, B)
         if self.fused and self.net.fused_ok():
-            self.net.forward_fused(B)
+            self.net.forward_graph(B)
         else:
             self.net.forward(B)
         return self.apply(psi_c, B, cbed)
@@ -733,12 +773,21 @@ def make_spectral_pools(seeds, factors=(8,), probes=4, grid=256, pix=0.125, kv=2
     return pool.finalize(), mats
 
 
-def run_spectral(corr, mat, pos, thickness, chunk=None, vhat=None):
+DEFAULT_TAU = 0.25   # fall back to fine slicing where the network changes the CBED by more than 25 %
+
+
+def run_spectral(corr, mat, pos, thickness, chunk=None, vhat=None, tau=DEFAULT_TAU, stats=None):
     """The learned coarse path: CBEDs [B][Ny][Nx] of probes at pos through
     thickness / (factor dz) coarse slices, each probe's exit spectrum
-    corrected once by the network.  chunk: probes per network call (bounds
-    the activation memory); vhat: the potential spectrum over the thickness
-    (computed when None)."""
+    corrected once by the network.
+
+    Fallback: the network's residual estimate for a probe is the relative
+    change it makes to the coarse CBED, ||I_learned - I_coarse|| / ||I_learned||
+    over the window; probes above tau are recomputed by plain fine slicing
+    (tau None: never).  chunk: probes per network call (bounds the
+    activation memory); vhat: the potential spectrum over the thickness
+    (computed when None); stats: a dict that receives the estimates and the
+    fallback count."""
     torch = _torch()
     B = pos.shape[0]
     f = corr.factor
@@ -755,9 +804,34 @@ def run_spectral(corr, mat, pos, thickness, chunk=None, vhat=None):
     th = torch.full((chunk,), float(S_f * mat.dz), dtype=torch.float32, device=dev)
     pc = torch.empty((chunk,) + shape, dtype=torch.complex64, device=dev)
     p0 = torch.empty_like(pc)
+    est = torch.empty((B,), dtype=torch.float64, device=dev) if tau is not None else None
+    r0, c0, H, W = corr.window
     for s in range(0, B, chunk):
         b = min(chunk, B - s)
         mat.exit_spectra(pos[s:s + b], S_f // f, f, pc[:b])
         mat.vacuum_spectra(pos[s:s + b], S_f * mat.dz, p0[:b])
         corr.correct(pc[:b], p0[:b], vhat.view((1,) + shape), None, km[:b], th[:b], cb[s:s + b])
+        if tau is not None:
+            # with the correction zeroed the loss kernel's I is the coarse CBED; against the learned CBED
+            # as target it gives sum (I_coarse - I_learned)^2 and sum I_learned^2 per probe
+            net = corr.net
+            call("namsa_fill", ptr(net.out), 2 * b * H * W, np.float32(0.0), net.st)
+            call("namsa_spectral_loss", ptr(net.out), ptr(pc), ptr(cb[s]), b, mat.Ny, mat.Nx, r0, c0, H, W,
+                 np.float32(1.0), ptr(net.sums), None, net.st)
+            sums = net.sums[:2 * b].view(b, 2)
+            est[s:s + b] = (sums[:, 0] / sums[:, 1].clamp_min(1e-300)).sqrt()
+    nf = 0
+    if tau is not None:
+        e = est.cpu().numpy()
+        bad = np.nonzero(~(e <= tau))[0]          # NaN estimates fall back too
+        nf = len(bad)
+        if nf:
+            idx = torch.as_tensor(bad, device=dev)
+            fine = torch.empty((nf,) + shape, dtype=torch.float32, device=dev)
+            mat.cbed(pos.index_select(0, idx), S_f, None, fine)
+            cb.index_copy_(0, idx, fine)
+        if stats is not None:
+            stats["estimate"] = e
+    if stats is not None:
+        stats["fallbacks"] = nf
     return cb

@@ -92,6 +92,8 @@ __device__ __forceinline__ void store_z(float* __restrict__ Z, const f32x16& zac
 }
 
 // x0 = Wl u + bl (u channel-major [cin][rows * W]) and layer 0's row DFT.
+// Per 32-pixel tile one MFMA with K = 16 (cin inputs, the rest zero):
+// A(x, k) = u[k][x], B(k, o) = Wl[o][k].
 __global__ void __launch_bounds__(256)
 fno_lift_kernel(const float* __restrict__ u, int cin, const float* __restrict__ Wl, const float* __restrict__ bl,
                 const __bf16* __restrict__ fxT, RowGeom g, __bf16* __restrict__ xout, float* __restrict__ Z) {
@@ -101,34 +103,43 @@ fno_lift_kernel(const float* __restrict__ u, int cin, const float* __restrict__ 
   const bool live = r < g.rows;
   __bf16 (*tt)[kPitch] = tt_all[wave];
   const long long n = static_cast<long long>(g.rows) * g.W;
-  const int xl = lane & 31, half = lane >> 5;
+  const int col = lane & 31, kh = 8 * (lane >> 5);
+  bf16x8 wop;                                    // B(k = kh + e, o = col)
+#pragma unroll
+  for (int e = 0; e < 8; ++e) wop[e] = __bf16(kh + e < cin ? Wl[col * cin + kh + e] : 0.0f);
+  const float bo = bl[col];
   f32x16 zacc;
 #pragma unroll
   for (int i = 0; i < 16; ++i) zacc[i] = 0.0f;
   for (int x0 = 0; x0 < g.W; x0 += kTile) {
-    const int px = x0 + xl;
-    const bool valid = live && px < g.W;
-    float uv[8];
+    const int pa = x0 + col;                     // this lane's A row (pixel)
+    const bool va = live && pa < g.W;
+    bf16x8 a;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) uv[i] = (valid && i < cin) ? u[i * n + static_cast<long long>(r) * g.W + px] : 0.0f;
-    bf16x8 out0, out1;
+    for (int e = 0; e < 8; ++e)
+      a[e] = __bf16((va && kh + e < cin) ? u[(kh + e) * n + static_cast<long long>(r) * g.W + pa] : 0.0f);
+    f32x16 acc;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const int o = 16 * half + j;
-      float v = bl[o];
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, wop, acc, 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (i < cin) v += Wl[o * cin + i] * uv[i];
-      const __bf16 h = __bf16(valid ? v : 0.0f);
-      tt[o][xl] = h;
-      if (j < 8) out0[j] = h; else out1[j - 8] = h;
-    }
-    if (valid) {
-      __bf16* dst = xout + (static_cast<long long>(r) * g.W + px) * kCh + 16 * half;
-      *reinterpret_cast<bf16x8*>(dst) = out0;
-      *reinterpret_cast<bf16x8*>(dst + 8) = out1;
+    for (int i = 0; i < 16; ++i) {
+      const int x = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+      tt[col][x] = __bf16(x0 + x < g.W ? acc[i] + bo : 0.0f);
     }
     __syncthreads();
+    const int px = x0 + col;                     // store: lane -> pixel l%32, channels 16(l/32) .. +16
+    if (live && px < g.W) {
+      bf16x8 o0, o1;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        o0[e] = tt[16 * (lane >> 5) + e][col];
+        o1[e] = tt[16 * (lane >> 5) + 8 + e][col];
+      }
+      __bf16* dst = xout + (static_cast<long long>(r) * g.W + px) * kCh + 16 * (lane >> 5);
+      *reinterpret_cast<bf16x8*>(dst) = o0;
+      *reinterpret_cast<bf16x8*>(dst + 8) = o1;
+    }
     tile_row_dft(zacc, fxT, g.Wp, x0, tt, lane);
     __syncthreads();
   }
@@ -146,7 +157,8 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
                  const float* __restrict__ Q1, const float* __restrict__ bq1, const float* __restrict__ Q2,
                  const float* __restrict__ bq2, float* __restrict__ out) {
   __shared__ __align__(16) __bf16 tt_all[kWaves][kCh][kPitch];           // [c][x] (LAST: [x][c])
-  __shared__ float hs_all[LAST ? kWaves : 1][LAST ? kTile : 1][LAST ? kHid + 1 : 1];
+  constexpr int kHPitch = kHid + 8;                                      // 144-byte rows: 16-byte aligned
+  __shared__ __align__(16) __bf16 hb_all[LAST ? kWaves : 1][LAST ? kTile : 1][LAST ? kHPitch : 8];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = blockIdx.x * kWaves + wave;
   const bool live = r < g.rows;
@@ -165,11 +177,16 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
   bop[3] = cvt8(Wt + col * kCh + 16 + kh, 1.0f);
   const float bo = bias[col];
   bf16x8 q1op[2][2];                            // head operands B(k = c, j) = Q1[j][c], constant per wave
+  bf16x8 q2op[4];                               // B(k = j, t) = Q2[t][j] for t = col < 2, else 0
   if (LAST) {
 #pragma unroll
     for (int jb = 0; jb < 2; ++jb)
 #pragma unroll
       for (int s = 0; s < 2; ++s) q1op[jb][s] = cvt8(Q1 + (32 * jb + col) * kCh + 16 * s + kh, 1.0f);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) q2op[s][e] = __bf16(col < 2 ? Q2[col * kHid + 16 * s + kh + e] : 0.0f);
   }
 
   f32x16 zacc;
@@ -224,8 +241,8 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
       }
       tile_row_dft(zacc, fxT, g.Wp, x0, tt, lane);
     } else {
-      // head: h = gelu(Q1 x + bq1) as two 32-wide MFMA blocks, A(x, k = c) = tt[x][c]
-      float (*hs)[kHid + 1] = hs_all[wave];
+      // head: h = gelu(Q1 x + bq1) as two 32-wide MFMA blocks, A(x, k = c) = tt[x][c], kept bf16 in LDS
+      __bf16 (*hb)[kHPitch] = hb_all[wave];
 #pragma unroll
       for (int jb = 0; jb < 2; ++jb) {
         f32x16 hacc;
@@ -238,14 +255,27 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
         }
         const float bj = bq1[32 * jb + col];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) hs[8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)][32 * jb + col] = gelu_t(hacc[i] + bj);
+        for (int i = 0; i < 16; ++i)
+          hb[8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)][32 * jb + col] = __bf16(gelu_t(hacc[i] + bj));
       }
       __syncthreads();
-      // out[t][p] = bq2[t] + sum_j Q2[t][j] h[j]: lane -> pixel l%32, output t = l/32
-      const int t = lane >> 5, px = x0 + col;
-      float o = bq2[t];
-      for (int j = 0; j < kHid; ++j) o += Q2[t * kHid + j] * hs[col][j];
-      if (live && px < g.W) out[t * n + static_cast<long long>(r) * g.W + px] = o;
+      // out[t][p] = Q2[t] . h + bq2[t]: four MFMAs with K = 64, A(x, k = j) = hb[x][j]; columns t < 2 kept
+      f32x16 oacc;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) oacc[i] = 0.0f;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const bf16x8 a = ld8(&hb[col][16 * s + kh]);
+        oacc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, q2op[s], oacc, 0, 0, 0);
+      }
+      if (live && col < 2) {
+        const float bt = bq2[col];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int px = x0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+          if (px < g.W) out[col * n + static_cast<long long>(r) * g.W + px] = oacc[i] + bt;
+        }
+      }
     }
     __syncthreads();
   }

@@ -9,7 +9,8 @@
 // follows the matrix: the corrector's products are skinny (one side is a
 // 32-channel or 32-mode dimension), so N <= 32 takes 128 x 32 tiles, M <= 32
 // takes 32 x 128 tiles and the rest 64 x 64 -- no wavefront idles on a
-// 32-wide side.  Operand tiles of K = 32 are staged through LDS as bf16 with
+// 32-wide side -- and batches of many items that each fit one 32 x 32 block
+// take one wavefront per item.  Operand tiles of K = 32 are staged through LDS as bf16 with
 // k contiguous: the unit-stride axis is read with 16-byte loads where the
 // strides and the base pointer allow it, and the next K tile is loaded into
 // registers while the matrix cores work on the current one.  MFMA operands
@@ -201,6 +202,51 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
+// Batched products whose every matrix fits one MFMA block (M <= 32, N <= 32;
+// the corrector's mode-space y transforms, thousands of 32 x 32 x H items):
+// one wavefront per batch item, operands read straight into the MFMA layout,
+// K in steps of 16.  No LDS and no barriers, so a wavefront past the last
+// item simply leaves.
+__global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs g) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long b = blockIdx.x * 4LL + wave;
+  if (b >= g.batch) return;
+  const float* A = g.A + b * g.sAb;
+  const float* B = g.B + b * g.sBb;
+  const int row = lane & 31, kh = 8 * (lane >> 5);
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    bf16x8 a, bb;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + kh + e;
+      a[e] = __bf16((row < g.M && k < g.K) ? A[row * g.sAi + k * g.sAk] : 0.0f);
+      bb[e] = __bf16((row < g.N && k < g.K) ? B[k * g.sBk + row * g.sBj] : 0.0f);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+  }
+  float* C = g.C + b * g.sCb;
+  float* P = g.P ? g.P + b * g.sCb : nullptr;
+  const int j = row;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (i >= g.M || j >= g.N) continue;
+    float* c = C + i * g.sCi + j * g.sCj;
+    float v = g.alpha * acc[r];
+    if (g.beta != 0.0f) v += g.beta * *c;
+    if (g.bias_mode == 1) v += g.bias[i];
+    if (g.bias_mode == 2) v += g.bias[j];
+    if (g.gelu) {
+      P[i * g.sCi + j * g.sCj] = v;
+      v = gelu_tanh(v);
+    }
+    *c = v;
+  }
+}
+
 template <int WM, int WN>
 int launch_gemm(const GemmArgs& g, hipStream_t st) {
   constexpr int TM = 32 * WM, TN = 32 * WN;
@@ -233,6 +279,10 @@ int namsa_gemm_bf16(const float* A, long long sAb, long long sAi, long long sAk,
   GemmArgs g{A, sAb, sAi, sAk, B, sBb, sBk, sBj, C, sCb, sCi, sCj, M, N, K, batch, split_k,
              alpha, beta, bias, bias_mode, gelu, P};
   hipStream_t st = as_stream(stream);
+  if (M <= 32 && N <= 32 && split_k == 1 && batch >= 64) {
+    hipLaunchKernelGGL(gemm_small_kernel, dim3(static_cast<unsigned>((batch + 3) / 4)), dim3(256), 0, st, g);
+    return check_hip(hipGetLastError(), "gemm_small_kernel launch");
+  }
   if (N <= 32) return launch_gemm<4, 1>(g, st);
   if (M <= 32) return launch_gemm<1, 4>(g, st);
   return launch_gemm<2, 2>(g, st);

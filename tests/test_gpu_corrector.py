@@ -95,3 +95,30 @@ def test_fused_forward_matches_oracle(gpu, H, W, B):
          f"{_rel(fp32, ref):.2e}")
     assert np.isfinite(fused).all()
     assert e_ref < 3e-2 and e_fp32 < 3e-2
+
+
+def test_forward_graph_replays_match_eager(gpu):
+    """The captured fused forward: every replay equals the eager pass bit for
+    bit, with eager work and synchronisations between replays, new inputs
+    between replays, and a larger batch (which reallocates) invalidating and
+    recapturing the graph."""
+    import torch
+    rng = np.random.default_rng(8)
+    H, W = 48, 80
+    net = K.FnoNet(H, W, cin=7, C=32, L=4, m=16, Hq=64, seed=9)
+    p = net.params_numpy()
+    p["Q2"] = rng.standard_normal(p["Q2"].shape) / 8
+    net.load(p)
+    for B in (3, 3, 5):
+        n = B * H * W
+        for rep in range(3):
+            net._ensure_common(B)
+            u = torch.from_numpy(rng.standard_normal((7, n)).astype(np.float32)).cuda()
+            net.u[:, :n].copy_(u)
+            eager = net.forward_fused(B)[:, :n].clone()
+            _ = float(eager.sum())                      # eager work and a synchronisation
+            net.out.fill_(np.nan)
+            got = net.forward_graph(B)[:, :n]
+            torch.cuda.synchronize()
+            assert torch.equal(got, eager), (B, rep)
+        _say(f"  B={B}: 3 replays equal to the eager pass")

@@ -80,6 +80,8 @@ def test_gemm_layouts(gpu, M, N, K, batch, a_t, b_t):
     (128, 200, 96, 2, (128 * 96, 1, 128), (96 * 200, 200, 1), 0, 0),    # A unit stride along i
     (64, 96, 40, 1, (64 * 40, 40, 1), (40 * 96, 1, 40), 0, 0),          # B unit stride along k
     (300, 32, 200, 1, (300 * 200, 200, 1), (200 * 32, 32, 1), 1, 3),    # misaligned bases: element loads
+    (32, 32, 170, 128, (32 * 170, 170, 1), (170 * 32, 32, 1), 0, 0),    # one wavefront per batch item (y DFT)
+    (20, 17, 40, 70, (20 * 40, 40, 1), (40 * 17 + 1, 1, 40), 0, 0),     # ragged items, K not a multiple of 16
 ])
 def test_gemm_tile_shapes_and_vector_loads(gpu, M, N, K, batch, sA, sB, offA, offB):
     import torch

@@ -212,7 +212,7 @@ def test_trained_corrector_gains_on_held_out_structures(gpu):
         pos = torch.from_numpy(rng.uniform(-L / 4, L / 4, (4, 2)).astype(np.float32)).to(sim.device)
         fine = mat.cbed(pos, S, None, torch.empty((4, 256, 256), device=sim.device)).cpu().numpy()
         coarse = mat.cbed(pos, S // 8, 8, torch.empty((4, 256, 256), device=sim.device)).cpu().numpy()
-        learned = run_spectral(corr, mat, pos, float(S)).cpu().numpy()
+        learned = run_spectral(corr, mat, pos, float(S), tau=None).cpu().numpy()      # the network alone
         for b in range(4):
             n = np.linalg.norm(fine[b])
             e_c.append(np.linalg.norm(coarse[b] - fine[b]) / n)
@@ -223,3 +223,29 @@ def test_trained_corrector_gains_on_held_out_structures(gpu):
          f"learned mean {e_l.mean():.4f} median {np.median(e_l):.4f}; improved {frac:.0%}")
     assert e_l.mean() < 0.95 * e_c.mean()
     assert frac >= 0.7
+
+
+def test_fallback_to_fine_slicing(material):
+    """tau = 0 sends every probe to plain fine slicing (bit for bit the fine
+    CBED); a large tau keeps every learned CBED; the estimates are finite."""
+    import torch
+
+    from namsa_amd.corrector import SpectralCorrector, run_spectral
+
+    mat = material
+    dev = mat.sim.device
+    pos = torch.tensor([[0.0, 0.0], [2.0, -1.0], [-3.5, 0.5]], dtype=torch.float32, device=dev)
+    corr = SpectralCorrector(256, 256, mat.window, mat.sim.kmax, mat.lam, 8.0, factor=8, device=dev)
+    p = corr.net.params_numpy()
+    p["Q2"] = np.random.default_rng(2).standard_normal(p["Q2"].shape) * 0.01     # a small, nonzero correction
+    corr.net.load(p)
+    fine = mat.cbed(pos, 48, None, torch.empty((3, 256, 256), device=dev)).cpu().numpy()
+    st = {}
+    allfb = run_spectral(corr, mat, pos, 48.0, tau=0.0, stats=st).cpu().numpy()
+    assert st["fallbacks"] == 3 and np.array_equal(allfb, fine)
+    st2 = {}
+    learned = run_spectral(corr, mat, pos, 48.0, tau=1e9, stats=st2).cpu().numpy()
+    assert st2["fallbacks"] == 0 and np.isfinite(st2["estimate"]).all() and (st2["estimate"] > 0).all()
+    plain = run_spectral(corr, mat, pos, 48.0, tau=None).cpu().numpy()
+    assert np.array_equal(learned, plain) and not np.array_equal(learned, fine)
+    _say(f"  estimates {np.round(st2['estimate'], 4)}")
