@@ -209,18 +209,29 @@ struct BlockSync {
 // next exchange's barrier, which every reader of this one has passed.
 struct RowLds {
   static constexpr bool DOUBLE = false;
+  static constexpr int P = 0;  // no split form
   float2* base;
   __device__ __forceinline__ float2& at(int i) const { return base[i + (i >> 4)]; }
   __device__ __forceinline__ void flip() {}
   static constexpr int padded(int n) { return n + n / 16; }
 };
+// ColLds::at_split(i0, k) is at(i0 + k) for a k that is a multiple of the
+// padding period P or an i0 that is: the padding then splits into a lane part
+// and a constant part, so an unrolled loop over k addresses LDS as one lane
+// base plus immediate offsets (with at(), the compiler evaluates every padded
+// index separately and keeps them live across a persistent loop).
 template <int C, bool DB = false>
 struct ColLds {
+  static_assert(32 % C == 0, "column tile must divide the 32-float2 padding period");
   static constexpr bool DOUBLE = DB;
+  static constexpr int P = 32 / C;  // one pad every P indices
   float2* base;
   int c;
   int other;  // DOUBLE: offset of the other buffer (+size or -size)
   __device__ __forceinline__ float2& at(int i) const { return base[i * C + c + ((i * C) >> 5)]; }
+  __device__ __forceinline__ float2& at_split(int i0, int k) const {
+    return base[i0 * C + c + i0 / P + (k * C + k / P)];
+  }
   __device__ __forceinline__ void flip() {
     if constexpr (DB) {
       base += other;
@@ -442,14 +453,34 @@ struct Fft<N, E, Radices<Rs...>, CT> {
         const int j = t + T * q;
         const int k = j % NS;
         const int dst = (j / NS) * NS * R + k;
+        // split form: the stride NS is a multiple of the padding period, or
+        // NS = 1 and dst = j R is
+        constexpr int P = Lds::P;
+        constexpr bool SPLIT = P > 0 && (NS % P == 0 || (NS == 1 && R % P == 0));
+        int d = dst;
+        // Otherwise, where the layout has a split form, the R padded
+        // addresses are formed here each time (d is made opaque) rather than
+        // hoisted out of the caller's persistent loop into R live registers.
+        if constexpr (P > 0 && !SPLIT) asm volatile("" : "+v"(d));
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds.at(dst + r * NS) = a[r];
+        for (int r = 0; r < R; ++r) {
+          if constexpr (SPLIT)
+            lds.at_split(d, r * NS) = a[r];
+          else
+            lds.at(d + r * NS) = a[r];
+        }
       }
     }
     if constexpr (!LAST) {
       Sync::sync();
+      constexpr bool SPLIT = Lds::P > 0 && T % (Lds::P > 0 ? Lds::P : 1) == 0;
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = lds.at(t + T * m);
+      for (int m = 0; m < E; ++m) {
+        if constexpr (SPLIT)
+          v[m] = lds.at_split(t, T * m);
+        else
+          v[m] = lds.at(t + T * m);
+      }
       if constexpr (Lds::DOUBLE)
         lds.flip();
       else

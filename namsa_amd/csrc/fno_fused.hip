@@ -43,9 +43,20 @@ constexpr int kTile = 32;      // pixels per tile
 constexpr int kPitch = 40;     // LDS row pitch in bf16 (80 bytes: 16-byte aligned rows)
 constexpr int kWaves = 4;
 
+// tanh-GELU, x/2 (1 + tanh(u)) = x (1 - 1 / (exp(2u) + 1)), u = sqrt(2/pi) (x + 0.044715 x^3);
+// the exponential form differs from tanhf by ~1e-7, far below the bf16 rounding of its result
 __device__ __forceinline__ float gelu_t(float x) {
-  const float c = 0.7978845608028654f;  // sqrt(2/pi)
-  return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  return x - x / (__expf(2.0f * u) + 1.0f);
+}
+
+// The LDS tiles are private to a wavefront: its lanes exchange through them
+// in program order, so a wavefront-scope fence (no wait on other wavefronts)
+// is the only ordering the exchanges need.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
@@ -127,7 +138,7 @@ fno_lift_kernel(const float* __restrict__ u, int cin, const float* __restrict__ 
       const int x = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
       tt[col][x] = __bf16(x0 + x < g.W ? acc[i] + bo : 0.0f);
     }
-    __syncthreads();
+    wave_sync();
     const int px = x0 + col;                     // store: lane -> pixel l%32, channels 16(l/32) .. +16
     if (live && px < g.W) {
       bf16x8 o0, o1;
@@ -141,7 +152,7 @@ fno_lift_kernel(const float* __restrict__ u, int cin, const float* __restrict__ 
       *reinterpret_cast<bf16x8*>(dst + 8) = o1;
     }
     tile_row_dft(zacc, fxT, g.Wp, x0, tt, lane);
-    __syncthreads();
+    wave_sync();
   }
   if (live) store_z(Z, zacc, g, r, lane);
 }
@@ -193,8 +204,25 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
 #pragma unroll
   for (int i = 0; i < 16; ++i) zacc[i] = 0.0f;
   const long long n = static_cast<long long>(g.rows) * g.W;
+  // channel operands A(x, k = 32 + 16 s + kh ..) of a tile; the next tile's are loaded while this one computes
+  auto load_ch = [&](int x0, bf16x8 (&a)[2]) {
+    const int pa = x0 + col;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (pa < g.W) {
+        a[s] = ld8(x_in + (static_cast<long long>(rr) * g.W + pa) * kCh + 16 * s + kh);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[s][e] = __bf16(0.0f);
+      }
+    }
+  };
+  bf16x8 anext[2];
+  load_ch(0, anext);
   for (int x0 = 0; x0 < g.W; x0 += kTile) {
     const int pa = x0 + col;                    // this lane's A row (pixel)
+    bf16x8 ach[2] = {anext[0], anext[1]};
+    if (x0 + kTile < g.W) load_ch(x0 + kTile, anext);
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
@@ -204,16 +232,8 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bop[s], acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {               // k = 32 + 16 s + kh .. : channels
-      bf16x8 a;
-      if (pa < g.W) {
-        a = ld8(x_in + (static_cast<long long>(rr) * g.W + pa) * kCh + 16 * s + kh);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) a[e] = __bf16(0.0f);
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bop[2 + s], acc, 0, 0, 0);
-    }
+    for (int s = 0; s < 2; ++s)                 // k = 32 + 16 s + kh .. : channels
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ach[s], bop[2 + s], acc, 0, 0, 0);
     // epilogue: pixel x = 8(i/4) + 4(l/32) + i%4 of the tile, channel o = col
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
@@ -224,7 +244,7 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
       if (LAST) tt[x][col] = __bf16(v);
       else tt[col][x] = __bf16(v);
     }
-    __syncthreads();
+    wave_sync();
     if constexpr (!LAST) {
       // the tile leaves as the next activation: lane -> pixel l%32, channels 16(l/32) .. +16
       const int px = x0 + col;
@@ -258,7 +278,7 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
         for (int i = 0; i < 16; ++i)
           hb[8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)][32 * jb + col] = __bf16(gelu_t(hacc[i] + bj));
       }
-      __syncthreads();
+      wave_sync();
       // out[t][p] = Q2[t] . h + bq2[t]: four MFMAs with K = 64, A(x, k = j) = hb[x][j]; columns t < 2 kept
       f32x16 oacc;
 #pragma unroll
@@ -277,7 +297,7 @@ fno_layer_kernel(const __bf16* __restrict__ x_in, const float* __restrict__ Zp, 
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   if constexpr (!LAST) {
     if (live) store_z(Z_next, zacc, g, r, lane);

@@ -40,14 +40,16 @@ static const SchedTable* table_for(int n) {
 // compact-twiddle (.ct) ones do (2-3% faster than the full ones), and the
 // fused driver switches those to the full ones for thick runs (see
 // precise_slices), where the compact forms' round-off would approach 1e-4.
-// The column pass double-buffers its LDS exchanges (one barrier per
-// exchange instead of two) at 384^2-768^2 and holds every twiddle (its
-// compact forms are 1-2% slower with twiddles loaded from tables), except at
-// 512^2, where only the compact form makes room for the second buffer
-// without spilling.
+// The column pass runs several workgroups per CU with one (strip, image)
+// task each (.pg.sb.w<k>: propagator read at use, scalar-base addressing,
+// register budget) at 128^2, 384^2-2048^2: 4-24% faster per slice step than
+// the persistent one-strip-per-workgroup schedules, which stay the defaults
+// at 192^2 and 256^2.  Thick runs at 768^2 take the double-buffered
+// persistent schedule, 5% faster there than the multi-workgroup one holding
+// every twiddle.
 //                                       128 192 256 384 512 768 1024 2048
 static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  6,  2,  3,  2};
-static const int kDefaultCol[kNumSizes] = { 0,  0,  2,  3, 14,  3,  1,  2};
+static const int kDefaultCol[kNumSizes] = { 2,  1,  2,  5, 15,  5,  5,  4};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];
@@ -67,9 +69,15 @@ bool supported(int n) { return size_index(n) >= 0; }
 
 // The schedule holding every twiddle that a compact-twiddle ("<name>.ct")
 // schedule is derived from (the base schedule 0 if it has no such partner),
+// the partner its entry names explicitly (where the derived one is slower),
 // or the schedule itself.
 template <class Entry>
 static int precise_partner(const Entry* list, int count, int sel) {
+  if (const char* named = list[sel].precise) {
+    for (int v = 0; v < count; ++v)
+      if (std::strcmp(list[v].name, named) == 0) return v;
+    return 0;
+  }
   const char* name = list[sel].name;
   const size_t len = std::strlen(name);
   if (len < 3 || std::strcmp(name + len - 3, ".ct") != 0) return sel;

@@ -61,14 +61,45 @@ struct RowS {
   static constexpr bool PF = PF_, NT = NT_, CT = CT_;
 };
 
-template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false, bool CT_ = false>
+// Column schedules also take W (a register budget for W wavefronts per SIMD,
+// e.g. two 512-thread workgroups per CU at W = 4; the launcher then gives each
+// workgroup one (strip, image) task), PG (PROP reads the propagator at each
+// use instead of holding the strip's values in registers) and SB (scalar
+// bases: element (jt + T m, col) is addressed as a wavefront-uniform base --
+// plane plus m T rows, in scalar registers -- plus one 32-bit lane offset,
+// instead of E 64-bit vector addresses that the compiler keeps live across
+// the persistent loop: 64 VGPRs at E = 16 for the loads and the stores).
+template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false, bool CT_ = false, int W_ = 0,
+          bool PG_ = false, bool SB_ = false>
 struct ColS {
   static_assert(C_ >= kMinColTile && C_ <= kMaxColTile, "column tile width out of range");
-  static constexpr int N = N_, E = E_, C = C_;
+  static constexpr int N = N_, E = E_, C = C_, W = W_;
   using RL = RL_;
-  static constexpr bool PF = PF_, NT = NT_, DB = DB_, CT = CT_;
+  static constexpr bool PF = PF_, NT = NT_, DB = DB_, CT = CT_, PG = PG_, SB = SB_;
   static constexpr int threads() { return C * (N / E); }
 };
+
+// Raw buffer access to one [Ny][Nx] complex64 plane: the resource (plane
+// base) and the byte offset of a row block are wavefront-uniform and live in
+// scalar registers, the lane's byte offset in one VGPR.  Written as buffer
+// instructions the compiler keeps this form; with plain pointers it folds the
+// uniform parts into per-element 64-bit vector addresses.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kBufDword3 = 0x00020000;  // gfx9 raw-buffer resource word 3
+constexpr int kBufNt = 2, kBufSc1 = 16;  // cache-policy bits of the aux operand
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float2* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(base), 0, static_cast<int>(bytes), kBufDword3);
+}
+template <bool NT>
+__device__ __forceinline__ float2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, NT ? kBufNt : 0));
+}
+// NT: non-temporal; otherwise sc1, the device-scope store of st<false>
+template <bool NT>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2 v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, NT ? kBufNt : kBufSc1);
+}
 
 // exp(2 pi i * turns) with the phase reduced to [-1/2, 1/2] turns in double.
 __device__ __forceinline__ float2 unit_phase(double turns) {
@@ -170,7 +201,7 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
 
 // ------------------------------------------------------------------ column pass
 template <class S, int OP>
-__global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
+__global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(ColArgs a) {
   constexpr int N = S::N, E = S::E, C = S::C;
   using F = Fft<N, E, typename S::RL, S::CT>;
   constexpr int T = F::T;
@@ -209,12 +240,24 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
   const bool live = OP != COL_READOUT || !a.apply_pm ||
                     (strip * C < a.band_hi && strip * C + C > a.band_lo);
 
-  float2 cst[E];  // PROP / READOUT: propagator * mask; PROBE: psi_k * (-1)^(row+col)
-  if constexpr (OP == COL_PROP || OP == COL_READOUT) {
+  // SB addressing: element (jt + T m, col) of a plane is at byte lane_off + m * row_block
+  // (the launcher checks that a plane's bytes fit 32 bits)
+  const unsigned lane_off = static_cast<unsigned>((jt * stride + col) * static_cast<long long>(sizeof(float2)));
+  const unsigned row_block = static_cast<unsigned>(T * stride * static_cast<long long>(sizeof(float2)));
+  const unsigned plane_bytes = static_cast<unsigned>(plane * static_cast<long long>(sizeof(float2)));
+  auto pm_at = [&](int m) -> float2 {
+    if constexpr (S::SB) return buf_ld<false>(plane_rsrc(a.pm, plane_bytes), lane_off, m * row_block);
+    else return a.pm[(jt + T * m) * stride + col];
+  };
+
+  constexpr bool PM_AT_USE = OP == COL_PROP && S::PG;
+  float2 cst[PM_AT_USE ? 1 : E];  // PROP / READOUT: propagator * mask; PROBE: psi_k * (-1)^(row+col)
+  if constexpr (PM_AT_USE) {
+  } else if constexpr (OP == COL_PROP || OP == COL_READOUT) {
     if (OP == COL_PROP || a.apply_pm) {
       const float sc = OP == COL_PROP ? a.pm_scale : 1.0f;
 #pragma unroll
-      for (int m = 0; m < E; ++m) cst[m] = cscale(a.pm[(jt + T * m) * stride + col], sc);
+      for (int m = 0; m < E; ++m) cst[m] = cscale(pm_at(m), sc);
     }
   } else if constexpr (OP == COL_PROBE) {
 #pragma unroll
@@ -226,9 +269,15 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
   }
 
   auto load = [&](long long b, float2(&v)[E]) {
-    const float2* p = a.psi + b * plane + col;
+    if constexpr (S::SB) {
+      const auto img = plane_rsrc(a.psi + b * plane, plane_bytes);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = ld<S::NT>(p + (jt + T * m) * stride);
+      for (int m = 0; m < E; ++m) v[m] = buf_ld<S::NT>(img, lane_off, m * row_block);
+    } else {
+      const float2* p = a.psi + b * plane + col;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = ld<S::NT>(p + (jt + T * m) * stride);
+    }
   };
 
   float2 pv[E];
@@ -264,8 +313,13 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
       if constexpr (OP == COL_PROP) {
         // compact twiddles: rewritten in place once per transform (fewer moves)
         F::template run<false, BlockSync, true>(v, lds, jt, tw);
+        if constexpr (PM_AT_USE) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cst[m]);
+          for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cscale(pm_at(m), a.pm_scale));
+        } else {
+#pragma unroll
+          for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cst[m]);
+        }
         F::template run<true, BlockSync, true>(v, lds, jt, tw);
       } else if constexpr (OP == COL_FWD || OP == COL_READOUT) {
         F::template run<false, BlockSync>(v, lds, jt, tw);
@@ -320,6 +374,10 @@ __global__ void __launch_bounds__(S::threads()) col_kernel(ColArgs a) {
         }
         __syncthreads();
       }
+    } else if constexpr (S::SB) {
+      const auto img = plane_rsrc(a.psi + b * plane, plane_bytes);
+#pragma unroll
+      for (int m = 0; m < E; ++m) buf_st<S::NT>(img, lane_off, m * row_block, v[m]);
     } else {
 #pragma unroll
       for (int m = 0; m < E; ++m) st<S::NT>(base + (jt + T * m) * stride, v[m]);
@@ -352,6 +410,8 @@ template <class S, int OP>
 int launch_col_op(ColArgs& a, hipStream_t st) {
   auto k = col_kernel<S, OP>;
   static const int cap = persistent_cap(k, S::threads());
+  if (S::SB && static_cast<long long>(S::N) * a.Nx * static_cast<long long>(sizeof(float2)) > 0x7fffffffLL)
+    return fail(kErrArg, "column pass: a %d x %d plane exceeds the 32-bit offsets of this schedule", S::N, a.Nx);
   a.strips_per_image = a.Nx / S::C;
   if (OP == COL_PROP) {  // only the strips the propagator band reaches
     a.strip0 = a.band_lo / S::C;
@@ -359,6 +419,15 @@ int launch_col_op(ColArgs& a, hipStream_t st) {
   } else {
     a.strip0 = 0;
     a.n_strips = a.strips_per_image;
+  }
+  if constexpr (S::W > 0) {
+    // Schedules sized for several workgroups per CU: one (strip, image) task
+    // per workgroup, which the dispatcher deals out to the CU slots as they
+    // free up, so the strips x images tasks balance whatever their count.
+    const long long grid = static_cast<long long>(a.n_strips) * a.B;
+    if (grid > 0x7fffffffLL) return fail(kErrArg, "column pass: %lld tasks in one launch", grid);
+    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(grid)), dim3(S::threads()), 0, st, a);
+    return check_hip(hipGetLastError(), "column pass launch");
   }
   // Every workgroup keeps one strip for the whole launch, so a launch covers
   // a whole number of strip sets.  With more strips than resident workgroups
@@ -384,11 +453,13 @@ using ColLaunch = int (*)(int op, ColArgs&, hipStream_t);
 struct RowSchedEntry {
   const char* name;
   RowLaunch launch;
+  const char* precise = nullptr;  // thick-run partner when not derived from the name
 };
 struct ColSchedEntry {
   const char* name;
   ColLaunch launch;
   int C;
+  const char* precise = nullptr;  // thick-run partner when not derived from the name
 };
 struct SchedTable {
   const RowSchedEntry* rows;

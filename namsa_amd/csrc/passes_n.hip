@@ -17,12 +17,14 @@ using R0 = RowS<128, 8, Radices<8, 4, 4>, false, false>;
 using R1 = RowS<128, 8, Radices<8, 4, 4>, true, false>;
 using C0 = ColS<128, 8, Radices<8, 4, 4>, 16, false, false>;
 using C1 = ColS<128, 8, Radices<8, 4, 4>, 8, true, false>;
+using C2 = ColS<128, 8, Radices<8, 4, 4>, 16, false, false, false, false, 4, true, true>;
 using R2 = RowS<128, 8, Radices<8, 4, 4>, false, false, false, 6>;
 static const RowSchedEntry kRows[] = {{"e8.r844", launch_row<R0, R0>},
                                       {"e8.r844.pf", launch_row<R1, R0>},
                                       {"e8.r844.w6", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e8.r844.c16", launch_col<C0, C0>, 16},
-                                      {"e8.r844.c8.pf", launch_col<C1, C0>, 8}};
+                                      {"e8.r844.c8.pf", launch_col<C1, C0>, 8},
+                                      {"e8.r844.c16.pg.sb.w4", launch_col<C2, C0>, 16}};
 #elif NAMSA_PASS_N == 192
 using R0 = RowS<192, 12, Radices<3, 4, 4, 4>, false, false>;
 using R1 = RowS<192, 12, Radices<3, 4, 4, 4>, true, false>;
@@ -54,13 +56,17 @@ using C1 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false>;
 using R2 = RowS<384, 12, Radices<3, 2, 4, 4, 4>, false, false, true>;
 using C2 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, false, true>;
 using C3 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, true>;
+using C4 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
+using C5 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
 static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>},
                                       {"e12.r32444.pf", launch_row<R1, R0>},
                                       {"e12.r32444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r32444.c16", launch_col<C0, C0>, 16},
                                       {"e12.r32444.c8.pf", launch_col<C1, C0>, 8},
                                       {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8},
-                                      {"e12.r32444.c8.db.pf", launch_col<C3, C0>, 8}};
+                                      {"e12.r32444.c8.db.pf", launch_col<C3, C0>, 8},
+                                      {"e12.r32444.c8.pg.sb.w4", launch_col<C4, C0>, 8},
+                                      {"e12.r32444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8}};
 #elif NAMSA_PASS_N == 512
 using R0 = RowS<512, 8, Radices<8, 8, 8>, false, false>;
 using R1 = RowS<512, 8, Radices<8, 8, 8>, true, false>;
@@ -82,6 +88,8 @@ using C11 = ColS<512, 16, Radices<8, 8, 8>, 16, true, false, true>;
 using C12 = ColS<512, 16, Radices<16, 16, 2>, 16, true, false>;
 using C13 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, false, true>;
 using C14 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, true, true>;
+using C15 = ColS<512, 8, Radices<8, 8, 8>, 8, false, false, false, true, 6, true, true>;
+using C16 = ColS<512, 8, Radices<8, 8, 8>, 8, false, false, false, false, 6, true, true>;
 using R5 = RowS<512, 8, Radices<8, 8, 8>, false, false, true>;
 using R6 = RowS<512, 8, Radices<8, 8, 8>, false, false, false, 5>;
 static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
@@ -105,7 +113,9 @@ static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e16.r888.c16.db.pf", launch_col<C11, C0>, 16},
                                       {"e16.r16.16.2.c16.pf", launch_col<C12, C0>, 16},
                                       {"e8.r888.c16.pf.ct", launch_col<C13, C0>, 16},
-                                      {"e8.r888.c16.db.pf.ct", launch_col<C14, C0>, 16}};
+                                      {"e8.r888.c16.db.pf.ct", launch_col<C14, C0>, 16},
+                                      {"e8.r888.c8.pg.sb.w6.ct", launch_col<C15, C0>, 8},
+                                      {"e8.r888.c8.pg.sb.w6", launch_col<C16, C0>, 8}};
 #elif NAMSA_PASS_N == 768
 using R0 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false>;
 using R1 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, true, false>;
@@ -114,13 +124,17 @@ using C1 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false>;
 using R2 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false, true>;
 using C2 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, false, true>;
 using C3 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, true>;
+using C4 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
+using C5 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
 static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>},
                                       {"e12.r34444.pf", launch_row<R1, R0>},
                                       {"e12.r34444.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
                                       {"e12.r34444.c8.pf", launch_col<C1, C0>, 8},
                                       {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8},
-                                      {"e12.r34444.c8.db.pf", launch_col<C3, C0>, 8}};
+                                      {"e12.r34444.c8.db.pf", launch_col<C3, C0>, 8},
+                                      {"e12.r34444.c8.pg.sb.w4", launch_col<C4, C0>, 8},
+                                      {"e12.r34444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8, "e12.r34444.c8.db.pf"}};
 #elif NAMSA_PASS_N == 1024
 using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
@@ -136,10 +150,14 @@ static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
                                       {"e16.r16.8.8.ct", launch_row<R3, R0>},
                                       {"e16.r16.8.8.w3", launch_row<R4, R0>}};
 using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
+using C4 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, false, 4, true, true>;
+using C5 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, true, 4, true, true>;
 static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
                                       {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
                                       {"e8.r8844.c8", launch_col<C2, C0>, 8},
-                                      {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8}};
+                                      {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8},
+                                      {"e16.r16.8.8.c8.pg.sb.w4", launch_col<C4, C0>, 8},
+                                      {"e16.r16.8.8.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8}};
 #elif NAMSA_PASS_N == 2048
 using R0 = RowS<2048, 16, Radices<16, 16, 8>, false, false>;
 using R1 = RowS<2048, 16, Radices<16, 16, 8>, true, false>;
@@ -148,13 +166,17 @@ using C1 = ColS<2048, 16, Radices<16, 16, 8>, 8, false, false>;
 using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
 using R2 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true>;
 using C3 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false, false, true>;
+using C4 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, false, 4, true, true>;
+using C5 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, true, 4, true, true>;
 static const RowSchedEntry kRows[] = {{"e16.r16.16.8", launch_row<R0, R0>},
                                       {"e16.r16.16.8.pf", launch_row<R1, R0>},
                                       {"e16.r16.16.8.ct", launch_row<R2, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r16.16.8.c4", launch_col<C0, C0>, 4},
                                       {"e16.r16.16.8.c8", launch_col<C1, C0>, 8},
                                       {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4},
-                                      {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4}};
+                                      {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4},
+                                      {"e16.r16.16.8.c4.pg.sb.w4", launch_col<C4, C0>, 4},
+                                      {"e16.r16.16.8.c4.pg.sb.w4.ct", launch_col<C5, C0>, 4}};
 #else
 #error "no schedules for this transform length"
 #endif

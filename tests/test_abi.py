@@ -75,18 +75,24 @@ def test_schedule_registry_host_side():
 
 def test_precise_schedule_policy():
     """Thick runs swap each compact-twiddle schedule for the schedule holding
-    every twiddle it derives from; other schedules map to themselves."""
+    every twiddle it derives from, or for the partner its entry names (768^2's
+    multi-workgroup column schedule); other schedules map to themselves."""
     from namsa_amd import autotune
 
     lib = L.load()
     assert lib.namsa_precise_slices(512) == 300 and lib.namsa_precise_slices(768) == 200
     assert lib.namsa_precise_slices(500) == -1
+    named = {(768, 1, "e12.r34444.c8.pg.sb.w4.ct"): "e12.r34444.c8.db.pf"}
     for n in (128, 192, 256, 384, 512, 768, 1024, 2048):
         for p in (0, 1):
             names = autotune.schedules(n, p)
             for v, name in enumerate(names):
                 q = lib.namsa_sched_precise(n, p, v)
-                expect = names.index(name[:-3]) if name.endswith(".ct") else v
+                if (n, p, name) in named:
+                    expect = names.index(named[(n, p, name)])
+                else:
+                    expect = names.index(name[:-3]) if name.endswith(".ct") else v
                 assert q == expect, (n, p, name, names[q])
+                assert not names[q].endswith(".ct"), (n, p, name, names[q])
     assert lib.namsa_sched_precise(512, 0, 99) < 0
     assert autotune.driver_schedules(512, 150) == (autotune.selected(512, 0), autotune.selected(512, 1))
