@@ -40,16 +40,20 @@ static const SchedTable* table_for(int n) {
 // compact-twiddle (.ct) ones do (2-3% faster than the full ones), and the
 // fused driver switches those to the full ones for thick runs (see
 // precise_slices), where the compact forms' round-off would approach 1e-4.
-// The column pass runs several workgroups per CU with one (strip, image)
-// task each (.pg.sb.w<k>: propagator read at use, scalar-base addressing,
-// register budget) at 128^2, 384^2-2048^2: 4-24% faster per slice step than
-// the persistent one-strip-per-workgroup schedules, which stay the defaults
-// at 192^2 and 256^2.  Thick runs at 768^2 take the double-buffered
-// persistent schedule, 5% faster there than the multi-workgroup one holding
-// every twiddle.
+// The column pass runs several workgroups per CU (.w<k>: register budget,
+// one multi-workgroup launch of (strip, image) tasks; .sb: scalar-base
+// addressing) at 128^2 and 384^2-2048^2, 4-24% faster per slice step than the
+// persistent one-strip-per-workgroup schedules, which stay the defaults at
+// 192^2 and 256^2.  Their compact forms keep the strip's propagator in
+// registers for several images (.i<k>) at 768^2-2048^2; the forms holding
+// every twiddle read it at each use instead (.pg), and at 768^2 thick runs
+// take the double-buffered persistent schedule, faster there than either.
+// At 2048^2 the row step holds each transmission row across the group's
+// images (.tr; 5-8% faster per slice step there, slower at smaller sizes,
+// where the slice rows stay cached).
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  6,  2,  3,  2};
-static const int kDefaultCol[kNumSizes] = { 2,  1,  2,  5, 15,  5,  5,  4};
+static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  6,  2,  3,  4};
+static const int kDefaultCol[kNumSizes] = { 2,  1,  2,  5, 15,  6,  6,  6};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];

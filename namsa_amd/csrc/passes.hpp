@@ -18,6 +18,7 @@ struct RowArgs {
   float scale;        // INV: output scale
   int band_lo, band_hi;    // columns loaded (the rest are zero on input)
   int store_lo, store_hi;  // columns stored (the rest are never read afterwards)
+  int image_chunk;         // STEP with a resident transmission row: images per task (set by the launcher)
 };
 
 // Column pass over columns of length N = Ny of B images [B][Ny][Nx].  The

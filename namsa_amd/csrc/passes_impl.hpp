@@ -54,11 +54,14 @@ __device__ __forceinline__ void st(float2* p, float2 v) {
 // W > 0 asks the compiler for at least W wavefronts per SIMD (a register
 // budget of 512 / W VGPRs): the row pass keeps most of its memory traffic
 // under its arithmetic only with enough wavefronts in flight.
-template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false, int W_ = 0>
+// TR: the slice step holds each transmission row across the images of the
+// group (row_step_tr_kernel) instead of reading it once per image.
+template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false, int W_ = 0, bool TR_ = false>
 struct RowS {
+  static_assert(!(PF_ && TR_), "the transmission-resident step does not prefetch");
   static constexpr int N = N_, E = E_, W = W_;
   using RL = RL_;
-  static constexpr bool PF = PF_, NT = NT_, CT = CT_;
+  static constexpr bool PF = PF_, NT = NT_, CT = CT_, TR = TR_;
 };
 
 // Column schedules also take W (a register budget for W wavefronts per SIMD,
@@ -70,10 +73,11 @@ struct RowS {
 // instead of E 64-bit vector addresses that the compiler keeps live across
 // the persistent loop: 64 VGPRs at E = 16 for the loads and the stores).
 template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false, bool CT_ = false, int W_ = 0,
-          bool PG_ = false, bool SB_ = false>
+          bool PG_ = false, bool SB_ = false, int K_ = 1>
 struct ColS {
   static_assert(C_ >= kMinColTile && C_ <= kMaxColTile, "column tile width out of range");
-  static constexpr int N = N_, E = E_, C = C_, W = W_;
+  static_assert(K_ == 1 || W_ > 0, "images per workgroup apply to the multi-workgroup launch");
+  static constexpr int N = N_, E = E_, C = C_, W = W_, K = K_;
   using RL = RL_;
   static constexpr bool PF = PF_, NT = NT_, DB = DB_, CT = CT_, PG = PG_, SB = SB_;
   static constexpr int threads() { return C * (N / E); }
@@ -194,6 +198,63 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
       for (int m = 0; m < E; ++m) {
         const int x = t + T * m;
         if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + T * m, v[m]);
+      }
+    }
+  }
+}
+
+// Slice step with the transmission row resident: a task is a row group yb
+// (slice rows yb * G + g, one per group of T threads) and a chunk of
+// image_chunk consecutive images of the probe group, which it runs in turn
+// with the slice row held in registers, so a slice row is read once per
+// chunk instead of once per image.
+template <class S>
+__global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_step_tr_kernel(RowArgs a) {
+  constexpr int N = S::N, E = S::E;
+  using F = Fft<N, E, typename S::RL, S::CT>;
+  constexpr int T = F::T;
+  constexpr int G = kRowThreads / T;
+  using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+  __shared__ float2 smem[G * RowLds::padded(N)];
+
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  RowLds lds{smem + g * RowLds::padded(N)};
+  float2 tw[F::NTW_ALLOC];
+  F::twiddles(t, tw);
+
+  const int rows_img = a.rows_per_image;
+  const int images = static_cast<int>(a.n_rows / rows_img);
+  const int n_groups = (rows_img + G - 1) / G;
+  const int chunks = (images + a.image_chunk - 1) / a.image_chunk;
+  for (int u = blockIdx.x; u < n_groups * chunks; u += gridDim.x) {
+    const int yb = u / chunks, b0 = (u % chunks) * a.image_chunk;
+    const int b1 = min(images, b0 + a.image_chunk);
+    // a tail group's surplus threads repeat the last row and store nothing
+    const bool own = yb * G + g < rows_img;
+    const int y = own ? yb * G + g : rows_img - 1;
+    float2 tv[E];
+    const float2* tr = a.tsl + static_cast<long long>(y) * N + t;
+#pragma unroll
+    for (int m = 0; m < E; ++m) tv[m] = tr[T * m];
+    for (int b = b0; b < b1; ++b) {
+      float2* p = a.psi + (static_cast<long long>(b) * rows_img + y) * N + t;
+      float2 v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int x = t + T * m;
+        v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + T * m) : make_float2(0.f, 0.f);
+      }
+      F::template run<true, Sync>(v, lds, t, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], tv[m]);
+      F::template run<false, Sync>(v, lds, t, tw);
+      if (own) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          const int x = t + T * m;
+          if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + T * m, v[m]);
+        }
       }
     }
   }
@@ -397,6 +458,21 @@ int persistent_cap(K kernel, int threads) {
 
 template <class S, int OP>
 int launch_row_op(const RowArgs& a, hipStream_t st) {
+  if constexpr (S::TR && OP == ROW_STEP) {
+    // image chunks sized so that there are about two tasks per resident workgroup
+    auto k = row_step_tr_kernel<S>;
+    constexpr int G = kRowThreads / (S::N / S::E);
+    static const int cap = persistent_cap(k, kRowThreads);
+    const int groups = (a.rows_per_image + G - 1) / G;
+    const int images = static_cast<int>(a.n_rows / a.rows_per_image);
+    const int chunks = std::max(1, std::min(images, (2 * cap + groups - 1) / groups));
+    RowArgs c = a;
+    c.image_chunk = (images + chunks - 1) / chunks;
+    const long long tasks = static_cast<long long>(groups) * ((images + c.image_chunk - 1) / c.image_chunk);
+    const int grid = static_cast<int>(std::max<long long>(1, std::min<long long>(tasks, cap)));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kRowThreads), 0, st, c);
+    return check_hip(hipGetLastError(), "row pass launch");
+  }
   auto k = row_kernel<S, OP>;
   constexpr int G = kRowThreads / (S::N / S::E);
   static const int cap = persistent_cap(k, kRowThreads);
@@ -424,7 +500,7 @@ int launch_col_op(ColArgs& a, hipStream_t st) {
     // Schedules sized for several workgroups per CU: one (strip, image) task
     // per workgroup, which the dispatcher deals out to the CU slots as they
     // free up, so the strips x images tasks balance whatever their count.
-    const long long grid = static_cast<long long>(a.n_strips) * a.B;
+    const long long grid = static_cast<long long>(a.n_strips) * ((a.B + S::K - 1) / S::K);
     if (grid > 0x7fffffffLL) return fail(kErrArg, "column pass: %lld tasks in one launch", grid);
     hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(grid)), dim3(S::threads()), 0, st, a);
     return check_hip(hipGetLastError(), "column pass launch");

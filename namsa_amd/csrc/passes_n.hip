@@ -126,6 +126,7 @@ using C2 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, false, true>;
 using C3 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, true>;
 using C4 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
 using C5 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
+using C6 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, true, 4, false, true, 2>;
 static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>},
                                       {"e12.r34444.pf", launch_row<R1, R0>},
                                       {"e12.r34444.ct", launch_row<R2, R0>}};
@@ -134,7 +135,8 @@ static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
                                       {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8},
                                       {"e12.r34444.c8.db.pf", launch_col<C3, C0>, 8},
                                       {"e12.r34444.c8.pg.sb.w4", launch_col<C4, C0>, 8},
-                                      {"e12.r34444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8, "e12.r34444.c8.db.pf"}};
+                                      {"e12.r34444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8, "e12.r34444.c8.db.pf"},
+                                      {"e12.r34444.c8.sb.w4.i2.ct", launch_col<C6, C0>, 8, "e12.r34444.c8.db.pf"}};
 #elif NAMSA_PASS_N == 1024
 using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
@@ -152,12 +154,14 @@ static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
 using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
 using C4 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, false, 4, true, true>;
 using C5 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, true, 4, true, true>;
+using C6 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, true, 4, false, true, 4>;
 static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
                                       {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
                                       {"e8.r8844.c8", launch_col<C2, C0>, 8},
                                       {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8},
                                       {"e16.r16.8.8.c8.pg.sb.w4", launch_col<C4, C0>, 8},
-                                      {"e16.r16.8.8.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8}};
+                                      {"e16.r16.8.8.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8},
+                                      {"e16.r16.8.8.c8.sb.w4.i4.ct", launch_col<C6, C0>, 8, "e16.r16.8.8.c8.pg.sb.w4"}};
 #elif NAMSA_PASS_N == 2048
 using R0 = RowS<2048, 16, Radices<16, 16, 8>, false, false>;
 using R1 = RowS<2048, 16, Radices<16, 16, 8>, true, false>;
@@ -165,18 +169,24 @@ using C0 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false>;
 using C1 = ColS<2048, 16, Radices<16, 16, 8>, 8, false, false>;
 using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
 using R2 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true>;
+using R3 = RowS<2048, 16, Radices<16, 16, 8>, false, false, false, 0, true>;
+using R4 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true, 0, true>;
 using C3 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false, false, true>;
 using C4 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, false, 4, true, true>;
 using C5 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, true, 4, true, true>;
+using C6 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, true, 4, false, true, 3>;
 static const RowSchedEntry kRows[] = {{"e16.r16.16.8", launch_row<R0, R0>},
                                       {"e16.r16.16.8.pf", launch_row<R1, R0>},
-                                      {"e16.r16.16.8.ct", launch_row<R2, R0>}};
+                                      {"e16.r16.16.8.ct", launch_row<R2, R0>},
+                                      {"e16.r16.16.8.tr", launch_row<R3, R0>},
+                                      {"e16.r16.16.8.tr.ct", launch_row<R4, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r16.16.8.c4", launch_col<C0, C0>, 4},
                                       {"e16.r16.16.8.c8", launch_col<C1, C0>, 8},
                                       {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4},
                                       {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4},
                                       {"e16.r16.16.8.c4.pg.sb.w4", launch_col<C4, C0>, 4},
-                                      {"e16.r16.16.8.c4.pg.sb.w4.ct", launch_col<C5, C0>, 4}};
+                                      {"e16.r16.16.8.c4.pg.sb.w4.ct", launch_col<C5, C0>, 4},
+                                      {"e16.r16.16.8.c4.sb.w4.i3.ct", launch_col<C6, C0>, 4, "e16.r16.16.8.c4.pg.sb.w4"}};
 #else
 #error "no schedules for this transform length"
 #endif

@@ -75,14 +75,17 @@ def test_schedule_registry_host_side():
 
 def test_precise_schedule_policy():
     """Thick runs swap each compact-twiddle schedule for the schedule holding
-    every twiddle it derives from, or for the partner its entry names (768^2's
-    multi-workgroup column schedule); other schedules map to themselves."""
+    every twiddle it derives from, or for the partner its entry names (the
+    multi-image column schedules); other schedules map to themselves."""
     from namsa_amd import autotune
 
     lib = L.load()
     assert lib.namsa_precise_slices(512) == 300 and lib.namsa_precise_slices(768) == 200
     assert lib.namsa_precise_slices(500) == -1
-    named = {(768, 1, "e12.r34444.c8.pg.sb.w4.ct"): "e12.r34444.c8.db.pf"}
+    named = {(768, 1, "e12.r34444.c8.pg.sb.w4.ct"): "e12.r34444.c8.db.pf",
+             (768, 1, "e12.r34444.c8.sb.w4.i2.ct"): "e12.r34444.c8.db.pf",
+             (1024, 1, "e16.r16.8.8.c8.sb.w4.i4.ct"): "e16.r16.8.8.c8.pg.sb.w4",
+             (2048, 1, "e16.r16.16.8.c4.sb.w4.i3.ct"): "e16.r16.16.8.c4.pg.sb.w4"}
     for n in (128, 192, 256, 384, 512, 768, 1024, 2048):
         for p in (0, 1):
             names = autotune.schedules(n, p)
