@@ -57,7 +57,9 @@ def build_probe_cpu(kmax, Ny, Nx, lam, semi_angle, pos, smooth_apert=True,
     k_rad = np.sqrt(k_x ** 2 + k_y ** 2)
     k_semi = semi_angle / lam
     if smooth_apert:
-        apert = 1 / (1 + np.exp(-2 * apert_smooth * (k_semi - k_rad)))
+        # exp overflows to inf far outside the aperture, where 1 / (1 + inf) = 0 is the value wanted
+        with np.errstate(over="ignore"):
+            apert = 1 / (1 + np.exp(-2 * apert_smooth * (k_semi - k_rad)))
     else:
         apert = np.heaviside(k_semi - k_rad, 0.5)
     if scherzer and C3 > 1.0:
