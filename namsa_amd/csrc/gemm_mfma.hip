@@ -202,37 +202,17 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
-// Batched products whose every matrix fits one MFMA block (M <= 32, N <= 32;
-// the corrector's mode-space y transforms, thousands of 32 x 32 x H items):
-// one wavefront per batch item, operands read straight into the MFMA layout,
-// K in steps of 16.  No LDS and no barriers, so a wavefront past the last
-// item simply leaves.
-__global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs g) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long b = blockIdx.x * 4LL + wave;
-  if (b >= g.batch) return;
-  const float* A = g.A + b * g.sAb;
-  const float* B = g.B + b * g.sBb;
-  const int row = lane & 31, kh = 8 * (lane >> 5);
-  f32x16 acc;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    bf16x8 a, bb;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + kh + e;
-      a[e] = __bf16((row < g.M && k < g.K) ? A[row * g.sAi + k * g.sAk] : 0.0f);
-      bb[e] = __bf16((row < g.N && k < g.K) ? B[k * g.sBk + row * g.sBj] : 0.0f);
-    }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
-  }
+// Epilogue of one 32 x 32 MFMA block of item b, rows i0 .. i0 + 31:
+// C = alpha acc + beta C (+ bias), optionally keeping the pre-activation in P
+// and applying GELU.  Register r of lane l holds row 8(r/4) + 4(l/32) + r%4,
+// column l%32.
+__device__ __forceinline__ void store_block(const GemmArgs& g, long long b, const f32x16& acc, int i0, int lane) {
   float* C = g.C + b * g.sCb;
   float* P = g.P ? g.P + b * g.sCb : nullptr;
-  const int j = row;
+  const int j = lane & 31;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int i = i0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     if (i >= g.M || j >= g.N) continue;
     float* c = C + i * g.sCi + j * g.sCj;
     float v = g.alpha * acc[r];
@@ -245,6 +225,44 @@ __global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs g) {
     }
     *c = v;
   }
+}
+
+// Batched products whose every matrix fits one MFMA block (M <= 32, N <= 32;
+// the corrector's mode-space y transforms, thousands of 32 x 32 x H items):
+// one workgroup per batch item, its four wavefronts taking every fourth
+// K step of 16 with the operands read straight into the MFMA layout, the
+// partial accumulators summed through LDS.  Splitting K shortens each
+// wavefront's chain of dependent loads and MFMAs fourfold; one wavefront
+// per item left the loads' latency exposed (about 0.8 TB/s on the operand).
+__global__ void __launch_bounds__(256) gemm_small_kernel(GemmArgs g) {
+  __shared__ float part[3][16][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long b = blockIdx.x;
+  const float* A = g.A + b * g.sAb;
+  const float* B = g.B + b * g.sBb;
+  const int row = lane & 31, kh = 8 * (lane >> 5);
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 16 * wave; k0 < g.K; k0 += 64) {
+    bf16x8 a, bb;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + kh + e;
+      a[e] = __bf16((row < g.M && k < g.K) ? A[row * g.sAi + k * g.sAk] : 0.0f);
+      bb[e] = __bf16((row < g.N && k < g.K) ? B[k * g.sBk + row * g.sBj] : 0.0f);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) part[wave - 1][r][lane] = acc[r];
+  }
+  __syncthreads();
+  if (wave > 0) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] += part[0][r][lane] + part[1][r][lane] + part[2][r][lane];
+  store_block(g, b, acc, 0, lane);
 }
 
 template <int WM, int WN>
@@ -280,7 +298,7 @@ int namsa_gemm_bf16(const float* A, long long sAb, long long sAi, long long sAk,
              alpha, beta, bias, bias_mode, gelu, P};
   hipStream_t st = as_stream(stream);
   if (M <= 32 && N <= 32 && split_k == 1 && batch >= 64) {
-    hipLaunchKernelGGL(gemm_small_kernel, dim3(static_cast<unsigned>((batch + 3) / 4)), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(gemm_small_kernel, dim3(static_cast<unsigned>(batch)), dim3(256), 0, st, g);
     return check_hip(hipGetLastError(), "gemm_small_kernel launch");
   }
   if (N <= 32) return launch_gemm<4, 1>(g, st);

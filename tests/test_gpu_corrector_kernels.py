@@ -80,8 +80,10 @@ def test_gemm_layouts(gpu, M, N, K, batch, a_t, b_t):
     (128, 200, 96, 2, (128 * 96, 1, 128), (96 * 200, 200, 1), 0, 0),    # A unit stride along i
     (64, 96, 40, 1, (64 * 40, 40, 1), (40 * 96, 1, 40), 0, 0),          # B unit stride along k
     (300, 32, 200, 1, (300 * 200, 200, 1), (200 * 32, 32, 1), 1, 3),    # misaligned bases: element loads
-    (32, 32, 170, 128, (32 * 170, 170, 1), (170 * 32, 32, 1), 0, 0),    # one wavefront per batch item (y DFT)
+    (32, 32, 170, 128, (32 * 170, 170, 1), (170 * 32, 32, 1), 0, 0),    # K split over four wavefronts (y DFT)
     (20, 17, 40, 70, (20 * 40, 40, 1), (40 * 17 + 1, 1, 40), 0, 0),     # ragged items, K not a multiple of 16
+    (20, 17, 8, 70, (20 * 8, 8, 1), (8 * 17, 17, 1), 0, 0),             # K below one step
+    (170, 32, 32, 96, (170 * 32, 1, 170), (32 * 32, 32, 1), 0, 0),      # thin-K batched items (inverse y DFT)
 ])
 def test_gemm_tile_shapes_and_vector_loads(gpu, M, N, K, batch, sA, sB, offA, offB):
     import torch
@@ -138,6 +140,22 @@ def test_gemm_epilogue_bias_beta_gelu_and_split_k(gpu):
     Cd = torch.from_numpy(C0.copy()).cuda()
     _gemm(torch, Ad, s[0], Bd, s[1], Cd, s[2], M, N, K, split=5, alpha=-1.0)
     np.testing.assert_allclose(Cd.cpu().numpy(), C0 - ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+    # the same epilogue on the batched small-item path (32 x 32 items) and on batched tiles
+    for m, n, k in ((32, 24, 100), (90, 30, 48)):
+        nb = 64
+        a = rng.standard_normal((nb, m, k)).astype(np.float32)
+        bm = rng.standard_normal((nb, k, n)).astype(np.float32) * 0.1
+        c0 = rng.standard_normal((nb, m, n)).astype(np.float32)
+        bi = rng.standard_normal(m).astype(np.float32)
+        Cd = torch.from_numpy(c0.copy()).cuda()
+        Pd = torch.zeros_like(Cd)
+        _gemm(torch, torch.from_numpy(a).cuda(), (m * k, k, 1), torch.from_numpy(bm).cuda(), (k * n, n, 1), Cd,
+              (m * n, n, 1), m, n, k, batch=nb, alpha=0.5, beta=2.0, bias=torch.from_numpy(bi).cuda(), bias_mode=1,
+              gelu=True, P=Pd)
+        pre = 0.5 * np.einsum("bik,bkj->bij", bf16(a).reshape(nb, m, k), bf16(bm).reshape(nb, k, n)) + 2.0 * c0 \
+            + bi[None, :, None]
+        np.testing.assert_allclose(Pd.cpu().numpy(), pre, rtol=0, atol=2e-5 * np.abs(pre).max())
+        np.testing.assert_allclose(Cd.cpu().numpy(), F.gelu(pre), rtol=0, atol=2e-5 * np.abs(pre).max())
 
 
 def test_gemm_rejects_bad_arguments(gpu):
