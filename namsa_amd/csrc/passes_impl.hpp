@@ -561,7 +561,14 @@ int launch_row(int op, const RowArgs& a, hipStream_t st) {
 template <class S, class Base>
 int launch_col(int op, ColArgs& a, hipStream_t st) {
   if (op == COL_PROP) return launch_col_op<S, COL_PROP>(a, st);
-  if (op == COL_READOUT) return launch_col_op<S, COL_READOUT>(a, st);
+  // the multi-workgroup schedules' readout (one task per (strip, image),
+  // detector reductions per task) is 20-50% slower than the base schedule's
+  if (op == COL_READOUT) {
+    if constexpr (S::W > 0)
+      return launch_col<Base, Base>(op, a, st);
+    else
+      return launch_col_op<S, COL_READOUT>(a, st);
+  }
   if constexpr (std::is_same<S, Base>::value) {
     if (op == COL_PROBE) return launch_col_op<S, COL_PROBE>(a, st);
     if (op == COL_FWD) return launch_col_op<S, COL_FWD>(a, st);
