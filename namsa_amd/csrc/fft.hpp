@@ -207,14 +207,19 @@ struct BlockSync {
 // buffer, so the next write cannot overtake a slow reader) or a switch to the
 // other of two buffers (DOUBLE): a later write to this buffer comes after the
 // next exchange's barrier, which every reader of this one has passed.
-struct RowLds {
+// SPLIT: stages may address it as at_split (see ColLds below), the padding
+// period being 16.
+template <bool SPLIT = false>
+struct RowLdsT {
   static constexpr bool DOUBLE = false;
-  static constexpr int P = 0;  // no split form
+  static constexpr int P = SPLIT ? 16 : 0;  // 0: stages use at() only
   float2* base;
   __device__ __forceinline__ float2& at(int i) const { return base[i + (i >> 4)]; }
+  __device__ __forceinline__ float2& at_split(int i0, int k) const { return base[i0 + i0 / 16 + (k + k / 16)]; }
   __device__ __forceinline__ void flip() {}
   static constexpr int padded(int n) { return n + n / 16; }
 };
+using RowLds = RowLdsT<false>;
 // ColLds::at_split(i0, k) is at(i0 + k) for a k that is a multiple of the
 // padding period P or an i0 that is: the padding then splits into a lane part
 // and a constant part, so an unrolled loop over k addresses LDS as one lane
@@ -227,7 +232,8 @@ struct ColLds {
   static constexpr int P = 32 / C;  // one pad every P indices
   float2* base;
   int c;
-  int other;  // DOUBLE: offset of the other buffer (+size or -size)
+  int other;                       // DOUBLE: offset of the other buffer (+size or -size)
+  const float2* roots = nullptr;   // LR transforms: exp(-2 pi i j / N), j < N/4 (Fft)
   __device__ __forceinline__ float2& at(int i) const { return base[i * C + c + ((i * C) >> 5)]; }
   __device__ __forceinline__ float2& at_split(int i0, int k) const {
     return base[i0 * C + c + i0 / P + (k * C + k / P)];
@@ -329,7 +335,15 @@ constexpr Pair unit_turns(long long m, long long d) {
 
 }  // namespace twc
 
-template <int N, int E, class RL, bool CT = false>
+// LR: the full twiddle set formed at each use from a quarter table of the
+// N-th roots of unity in LDS (ColLds::roots, filled by the kernel from
+// kRootsQ<N>): w = exp(-2 pi i e / N) is roots[e mod N/4] turned by the
+// exact quarter turn (-i)^(e div N/4).  unit_turns gives the same float pair
+// for an angle whatever the order it is written in (power-of-two scalings of
+// its integers are exact), and the quarter turns of the table reproduce it
+// (checked for every exponent at N = 2048), so LR transforms are bit-identical
+// to those holding the table's twiddles in registers, at no register cost.
+template <int N, int E, class RL, bool CT = false, bool LR = false>
 struct Fft;
 
 // A schedule's twiddle table (Fft::make_table), evaluated by the compiler and
@@ -337,9 +351,23 @@ struct Fft;
 template <class F>
 __device__ constexpr typename F::Table kTwTable = F::make_table();
 
-template <int N, int E, bool CT, int... Rs>
-struct Fft<N, E, Radices<Rs...>, CT> {
+template <int N>
+struct RootsQ {
+  twc::Pair v[N / 4];
+  static constexpr RootsQ make() {
+    RootsQ r{};
+    for (int j = 0; j < N / 4; ++j) r.v[j] = twc::unit_turns(-static_cast<long long>(j), N);
+    return r;
+  }
+};
+template <int N>
+__device__ constexpr RootsQ<N> kRootsQ = RootsQ<N>::make();
+
+template <int N, int E, bool CT, bool LR, int... Rs>
+struct Fft<N, E, Radices<Rs...>, CT, LR> {
   static_assert(Prod<Rs...>::value == N, "radix schedule must multiply to N");
+  static_assert(!(CT && LR), "LDS roots give the full twiddle set");
+  static_assert(!LR || (N & (N - 1)) == 0, "LDS roots need a power-of-two length");
   static constexpr int T = N / E;
   static constexpr int NTW = CT ? TwCountC<E, 1, Rs...>::value : TwCount<E, 1, Rs...>::value;
   // registers per butterfly of a twiddled stage
@@ -348,7 +376,8 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   // exponent of the r-th stored twiddle of a butterfly
   template <int R>
   static constexpr int stored_power(int r) { return CT && R >= 8 ? 1 << (r - 1) : r; }
-  static constexpr int NTW_ALLOC = NTW > 0 ? NTW : 1;
+  static constexpr int NTW_TAB = NTW > 0 ? NTW : 1;
+  static constexpr int NTW_ALLOC = LR ? 1 : NTW_TAB;  // twiddle registers a caller holds
   static constexpr int NSTAGES = sizeof...(Rs);
 
   // ---------------- twiddles (forward sign; inverse uses the conjugate)
@@ -356,7 +385,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   // (NS R)), k = (t + T q) mod NS, p = stored_power(r): the r-th stored power
   // for butterfly q of the stage whose sub-transforms have length NS.
   struct Table {
-    twc::Pair v[NTW_ALLOC * T];
+    twc::Pair v[NTW_TAB * T];
   };
 
   template <int NS, int OFF>
@@ -387,7 +416,7 @@ struct Fft<N, E, Radices<Rs...>, CT> {
   }
 
   static __device__ __forceinline__ void twiddles(int t, float2 (&tw)[NTW_ALLOC]) {
-    if constexpr (NTW > 0) {
+    if constexpr (NTW > 0 && !LR) {
       const twc::Pair* p = kTwTable<Fft>.v + t;
 #pragma unroll
       for (int i = 0; i < NTW; ++i) tw[i] = make_float2(p[i * T].c, p[i * T].s);
@@ -436,6 +465,23 @@ struct Fft<N, E, Radices<Rs...>, CT> {
           for (int r = 1; r < R; ++r) {
             a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
             if (r + 1 < R) w = cmul(w, w1);
+          }
+        } else if constexpr (LR) {
+          // w^r = exp(-2 pi i e / N), e = r k N / (NS R) mod N, k = (t + T q) mod NS; t is made
+          // opaque so that the exponents are formed here, not hoisted into registers
+          int tt = t;
+          asm volatile("" : "+v"(tt));
+          constexpr int SCALE = N / (NS * R), QN = N / 4;
+          const int step = ((tt + T * q) & (NS - 1)) * SCALE;
+          int e = 0;
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            e = (e + step) & (N - 1);
+            const float2 u = lds.roots[e & (QN - 1)];
+            const int quarter = e / QN;
+            float2 w = (quarter & 1) ? make_float2(u.y, -u.x) : u;
+            if (quarter & 2) w = make_float2(-w.x, -w.y);
+            a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
           }
         } else {
 #pragma unroll

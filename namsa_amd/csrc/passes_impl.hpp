@@ -55,14 +55,29 @@ __device__ __forceinline__ void st(float2* p, float2 v) {
 // budget of 512 / W VGPRs): the row pass keeps most of its memory traffic
 // under its arithmetic only with enough wavefronts in flight.
 // TR: the slice step holds each transmission row across the images of the
-// group (row_step_tr_kernel) instead of reading it once per image.
-template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false, int W_ = 0, bool TR_ = false>
+// group (row_step_tr_kernel) instead of reading it once per image.  LS: LDS
+// addressed as a lane base plus constant offsets where the padding allows
+// (RowLdsT<true>).
+template <int N_, int E_, class RL_, bool PF_, bool NT_, bool CT_ = false, int W_ = 0, bool TR_ = false,
+          bool LS_ = false>
 struct RowS {
   static_assert(!(PF_ && TR_), "the transmission-resident step does not prefetch");
   static constexpr int N = N_, E = E_, W = W_;
   using RL = RL_;
-  static constexpr bool PF = PF_, NT = NT_, CT = CT_, TR = TR_;
+  static constexpr bool PF = PF_, NT = NT_, CT = CT_, TR = TR_, LS = LS_;
 };
+
+// Fills the LDS quarter table of N-th roots an LR transform reads its
+// twiddles from, and points the LDS accessor at it (workgroup barrier).
+// Column schedules only: the row passes are VALU-bound, and forming the
+// twiddles there costs more than the registers it frees (5-10% slower at
+// 2048^2).
+template <int N, class Lds>
+__device__ __forceinline__ void load_roots(float2* roots, Lds& lds, int threads) {
+  for (int i = threadIdx.x; i < N / 4; i += threads) roots[i] = make_float2(kRootsQ<N>.v[i].c, kRootsQ<N>.v[i].s);
+  __syncthreads();
+  lds.roots = roots;
+}
 
 // Column schedules also take W (a register budget for W wavefronts per SIMD,
 // e.g. two 512-thread workgroups per CU at W = 4; the launcher then gives each
@@ -73,13 +88,13 @@ struct RowS {
 // instead of E 64-bit vector addresses that the compiler keeps live across
 // the persistent loop: 64 VGPRs at E = 16 for the loads and the stores).
 template <int N_, int E_, class RL_, int C_, bool PF_, bool NT_, bool DB_ = false, bool CT_ = false, int W_ = 0,
-          bool PG_ = false, bool SB_ = false, int K_ = 1>
+          bool PG_ = false, bool SB_ = false, int K_ = 1, bool LR_ = false>
 struct ColS {
   static_assert(C_ >= kMinColTile && C_ <= kMaxColTile, "column tile width out of range");
   static_assert(K_ == 1 || W_ > 0, "images per workgroup apply to the multi-workgroup launch");
   static constexpr int N = N_, E = E_, C = C_, W = W_, K = K_;
   using RL = RL_;
-  static constexpr bool PF = PF_, NT = NT_, DB = DB_, CT = CT_, PG = PG_, SB = SB_;
+  static constexpr bool PF = PF_, NT = NT_, DB = DB_, CT = CT_, PG = PG_, SB = SB_, LR = LR_;
   static constexpr int threads() { return C * (N / E); }
 };
 
@@ -122,11 +137,12 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
   constexpr int G = kRowThreads / T;
   constexpr bool STEP = OP == ROW_STEP;
   using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
-  __shared__ float2 smem[G * RowLds::padded(N)];
+  using Lds = RowLdsT<S::LS>;
+  __shared__ float2 smem[G * Lds::padded(N)];
 
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  RowLds lds{smem + g * RowLds::padded(N)};
+  Lds lds{smem + g * Lds::padded(N)};
   float2 tw[F::NTW_ALLOC];
   F::twiddles(t, tw);
 
@@ -215,11 +231,12 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_step_tr_
   constexpr int T = F::T;
   constexpr int G = kRowThreads / T;
   using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
-  __shared__ float2 smem[G * RowLds::padded(N)];
+  using Lds = RowLdsT<S::LS>;
+  __shared__ float2 smem[G * Lds::padded(N)];
 
   const int g = threadIdx.x / T;
   const int t = threadIdx.x % T;
-  RowLds lds{smem + g * RowLds::padded(N)};
+  Lds lds{smem + g * Lds::padded(N)};
   float2 tw[F::NTW_ALLOC];
   F::twiddles(t, tw);
 
@@ -264,7 +281,7 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_step_tr_
 template <class S, int OP>
 __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(ColArgs a) {
   constexpr int N = S::N, E = S::E, C = S::C;
-  using F = Fft<N, E, typename S::RL, S::CT>;
+  using F = Fft<N, E, typename S::RL, S::CT, S::LR>;
   constexpr int T = F::T;
   constexpr int NW = S::threads() / 64;
   constexpr bool LOADS = OP != COL_PROBE;
@@ -272,10 +289,12 @@ __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(
   constexpr int LDS_ONE = Lds::padded(N);
   __shared__ float2 smem[S::DB ? 2 * LDS_ONE : LDS_ONE];
   __shared__ double red[NW > 0 ? NW : 1];
+  __shared__ float2 roots[S::LR ? N / 4 : 1];
 
   const int c = threadIdx.x % C;
   const int jt = threadIdx.x / C;
   Lds lds{smem, c, LDS_ONE};
+  if constexpr (S::LR) load_roots<N>(roots, lds, S::threads());
   float2 tw[F::NTW_ALLOC];
   F::twiddles(jt, tw);
   const long long plane = static_cast<long long>(N) * a.Nx;

@@ -90,6 +90,7 @@ using C13 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, false, true>;
 using C14 = ColS<512, 8, Radices<8, 8, 8>, 16, true, false, true, true>;
 using C15 = ColS<512, 8, Radices<8, 8, 8>, 8, false, false, false, true, 6, true, true>;
 using C16 = ColS<512, 8, Radices<8, 8, 8>, 8, false, false, false, false, 6, true, true>;
+using C17 = ColS<512, 8, Radices<8, 8, 8>, 8, false, false, false, false, 6, true, true, 1, true>;
 using R5 = RowS<512, 8, Radices<8, 8, 8>, false, false, true>;
 using R6 = RowS<512, 8, Radices<8, 8, 8>, false, false, false, 5>;
 static const RowSchedEntry kRows[] = {{"e8.r888", launch_row<R0, R0>},
@@ -114,8 +115,9 @@ static const ColSchedEntry kCols[] = {{"e16.r888.c16", launch_col<C0, C0>, 16},
                                       {"e16.r16.16.2.c16.pf", launch_col<C12, C0>, 16},
                                       {"e8.r888.c16.pf.ct", launch_col<C13, C0>, 16},
                                       {"e8.r888.c16.db.pf.ct", launch_col<C14, C0>, 16},
-                                      {"e8.r888.c8.pg.sb.w6.ct", launch_col<C15, C0>, 8},
-                                      {"e8.r888.c8.pg.sb.w6", launch_col<C16, C0>, 8}};
+                                      {"e8.r888.c8.pg.sb.w6.ct", launch_col<C15, C0>, 8, "e8.r888.c8.lr.pg.sb.w6"},
+                                      {"e8.r888.c8.pg.sb.w6", launch_col<C16, C0>, 8},
+                                      {"e8.r888.c8.lr.pg.sb.w6", launch_col<C17, C0>, 8}};
 #elif NAMSA_PASS_N == 768
 using R0 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, false, false>;
 using R1 = RowS<768, 12, Radices<3, 4, 4, 4, 4>, true, false>;
@@ -155,13 +157,15 @@ using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
 using C4 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, false, 4, true, true>;
 using C5 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, true, 4, true, true>;
 using C6 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, true, 4, false, true, 4>;
+using C7 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, false, 4, true, true, 1, true>;
 static const ColSchedEntry kCols[] = {{"e16.r16.8.8.c8", launch_col<C0, C0>, 8},
                                       {"e16.r16.8.8.c8.pf", launch_col<C1, C0>, 8},
                                       {"e8.r8844.c8", launch_col<C2, C0>, 8},
                                       {"e16.r16.8.8.c8.pf.ct", launch_col<C3, C0>, 8},
                                       {"e16.r16.8.8.c8.pg.sb.w4", launch_col<C4, C0>, 8},
                                       {"e16.r16.8.8.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8},
-                                      {"e16.r16.8.8.c8.sb.w4.i4.ct", launch_col<C6, C0>, 8, "e16.r16.8.8.c8.pg.sb.w4"}};
+                                      {"e16.r16.8.8.c8.sb.w4.i4.ct", launch_col<C6, C0>, 8, "e16.r16.8.8.c8.lr.pg.sb.w4"},
+                                      {"e16.r16.8.8.c8.lr.pg.sb.w4", launch_col<C7, C0>, 8}};
 #elif NAMSA_PASS_N == 2048
 using R0 = RowS<2048, 16, Radices<16, 16, 8>, false, false>;
 using R1 = RowS<2048, 16, Radices<16, 16, 8>, true, false>;
@@ -171,22 +175,26 @@ using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
 using R2 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true>;
 using R3 = RowS<2048, 16, Radices<16, 16, 8>, false, false, false, 0, true>;
 using R4 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true, 0, true>;
+using R5 = RowS<2048, 16, Radices<16, 16, 8>, false, false, false, 0, true, true>;
 using C3 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false, false, true>;
 using C4 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, false, 4, true, true>;
 using C5 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, true, 4, true, true>;
 using C6 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, true, 4, false, true, 3>;
+using C7 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, false, 4, true, true, 1, true>;
 static const RowSchedEntry kRows[] = {{"e16.r16.16.8", launch_row<R0, R0>},
                                       {"e16.r16.16.8.pf", launch_row<R1, R0>},
                                       {"e16.r16.16.8.ct", launch_row<R2, R0>},
                                       {"e16.r16.16.8.tr", launch_row<R3, R0>},
-                                      {"e16.r16.16.8.tr.ct", launch_row<R4, R0>}};
+                                      {"e16.r16.16.8.tr.ct", launch_row<R4, R0>, "e16.r16.16.8.tr.ls"},
+                                      {"e16.r16.16.8.tr.ls", launch_row<R5, R0>}};
 static const ColSchedEntry kCols[] = {{"e16.r16.16.8.c4", launch_col<C0, C0>, 4},
                                       {"e16.r16.16.8.c8", launch_col<C1, C0>, 8},
                                       {"e16.r16.16.8.c4.pf", launch_col<C2, C0>, 4},
                                       {"e16.r16.16.8.c4.pf.ct", launch_col<C3, C0>, 4},
                                       {"e16.r16.16.8.c4.pg.sb.w4", launch_col<C4, C0>, 4},
                                       {"e16.r16.16.8.c4.pg.sb.w4.ct", launch_col<C5, C0>, 4},
-                                      {"e16.r16.16.8.c4.sb.w4.i3.ct", launch_col<C6, C0>, 4, "e16.r16.16.8.c4.pg.sb.w4"}};
+                                      {"e16.r16.16.8.c4.sb.w4.i3.ct", launch_col<C6, C0>, 4, "e16.r16.16.8.c4.lr.pg.sb.w4"},
+                                      {"e16.r16.16.8.c4.lr.pg.sb.w4", launch_col<C7, C0>, 4}};
 #else
 #error "no schedules for this transform length"
 #endif

@@ -76,7 +76,8 @@ def test_schedule_registry_host_side():
 def test_precise_schedule_policy():
     """Thick runs swap each compact-twiddle schedule for the schedule holding
     every twiddle it derives from, or for the partner its entry names (the
-    multi-image column schedules); other schedules map to themselves."""
+    multi-image column schedules, and the 2048 row and column schedules whose
+    exact partner is a different kernel); other schedules map to themselves."""
     from namsa_amd import autotune
 
     lib = L.load()
@@ -84,8 +85,10 @@ def test_precise_schedule_policy():
     assert lib.namsa_precise_slices(500) == -1
     named = {(768, 1, "e12.r34444.c8.pg.sb.w4.ct"): "e12.r34444.c8.db.pf",
              (768, 1, "e12.r34444.c8.sb.w4.i2.ct"): "e12.r34444.c8.db.pf",
-             (1024, 1, "e16.r16.8.8.c8.sb.w4.i4.ct"): "e16.r16.8.8.c8.pg.sb.w4",
-             (2048, 1, "e16.r16.16.8.c4.sb.w4.i3.ct"): "e16.r16.16.8.c4.pg.sb.w4"}
+             (512, 1, "e8.r888.c8.pg.sb.w6.ct"): "e8.r888.c8.lr.pg.sb.w6",
+             (1024, 1, "e16.r16.8.8.c8.sb.w4.i4.ct"): "e16.r16.8.8.c8.lr.pg.sb.w4",
+             (2048, 0, "e16.r16.16.8.tr.ct"): "e16.r16.16.8.tr.ls",
+             (2048, 1, "e16.r16.16.8.c4.sb.w4.i3.ct"): "e16.r16.16.8.c4.lr.pg.sb.w4"}
     for n in (128, 192, 256, 384, 512, 768, 1024, 2048):
         for p in (0, 1):
             names = autotune.schedules(n, p)

@@ -66,3 +66,32 @@ def test_every_schedule_matches_oracle(gpu, N):
     finally:
         autotune.select(N, 0, keep[0])
         autotune.select(N, 1, keep[1])
+
+
+@pytest.mark.parametrize("N,reg,lds", [(512, "e8.r888.c8.pg.sb.w6", "e8.r888.c8.lr.pg.sb.w6"),
+                                        (1024, "e16.r16.8.8.c8.pg.sb.w4", "e16.r16.8.8.c8.lr.pg.sb.w4"),
+                                        (2048, "e16.r16.16.8.c4.pg.sb.w4", "e16.r16.16.8.c4.lr.pg.sb.w4")])
+def test_lds_root_twiddles_match_register_twiddles(gpu, N, reg, lds):
+    """The exact twiddles formed from the LDS quarter table of roots (`.lr`
+    schedules) are the same float pairs as the register tables hold (checked
+    on the host for every exponent at 2048), so the two transforms differ only
+    where the compiler pairs a product differently inside a fused
+    multiply-add: float32 round-off, not a twiddle error (propagator read at
+    use in both)."""
+    import torch
+
+    psi_k, pos, t, pm, band, kmax, exit_ref, _ = _case(N, 7)
+    names = autotune.schedules(N, 1)
+    keep = autotune.selected(N, 1)
+    out = {}
+    try:
+        for name in (reg, lds):
+            autotune.select(N, 1, names.index(name))
+            print(f"N={N} column schedule={name}", flush=True)
+            out[name] = _run(torch, gpu, N, psi_k, pos, t, pm, band, kmax, L.OUT_EXIT)
+    finally:
+        autotune.select(N, 1, keep)
+    a, b = out.values()
+    d = rel_l2(b, a)
+    print(f"   register vs LDS-root twiddles: relL2 {d:.2e}; vs oracle {rel_l2(a, exit_ref):.2e} {rel_l2(b, exit_ref):.2e}")
+    assert d < 1e-6
