@@ -101,18 +101,21 @@ def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
     S = int(sim.num_slices)
     st = L.stream_handle()
     kmax = np.float32(sim.kmax)
+    # the wave layout namsa_multislice's CBED slice loop uses at this size
+    # (the probe former's row-major state only seeds finite values)
+    sfx = "_tiled" if L.load().namsa_wave_tiled(Ny, Nx) else ""
     L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), G, Ny, Nx, kmax, L.ptr(ws), st)
-    L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[0]), 0, Nx, st)
-    L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
+    L.call("namsa_slice_rows" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[0]), 0, Nx, st)
+    L.call("namsa_slice_cols" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
     reps = max(2, reps)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     ev[0].record()
     for r in range(reps):
-        L.call("namsa_slice_rows", L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[1 + r % max(1, S - 1)]), lo, hi, st)
+        L.call("namsa_slice_rows" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[1 + r % max(1, S - 1)]), lo, hi, st)
     ev[1].record()
     ev[2].record()
     for r in range(reps):
-        L.call("namsa_slice_cols", L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
+        L.call("namsa_slice_cols" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
     ev[3].record()
     torch.cuda.synchronize()
     row_ms = ev[0].elapsed_time(ev[1]) / reps
@@ -143,7 +146,8 @@ def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": d_bytes, "avg_launch_ms": d_ms,
             "launch_probes": G, "band_bytes_per_launch": d_band, "band_GBs": d_band / (d_ms * 1e-3) / 1e9,
             "band_frac": d_band / (d_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "slice_step_GBs": step_gbs,
-            "slice_step_band_GBs": step_band, "row_ms": row_ms, "col_ms": col_ms}
+            "slice_step_band_GBs": step_band, "row_ms": row_ms, "col_ms": col_ms,
+            "wave_layout": "2x4 tiles" if sfx else "row-major"}
 
 
 def host_cpu_info():

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define NAMSA_ABI_VERSION 2
+#define NAMSA_ABI_VERSION 3
 
 #define NAMSA_OK 0
 #define NAMSA_ERR_ARG (-1)
@@ -204,6 +204,22 @@ int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, i
                      int band_hi, void* stream);
 int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
                      void* stream);
+
+/*
+ * The same two passes over a wave stored in 2 x 4 tiles: element (y, x) of a
+ * [Ny][Nx] plane at ((y/2)(Nx/4) + x/4)*8 + (y%2)*4 + x%4.  In CBED mode
+ * namsa_multislice keeps its internal wave in this layout at the sizes where
+ * namsa_wave_tiled(Ny, Nx) returns 1 (2048 x 2048): the column pass then
+ * reads 64-byte pieces instead of 32-byte row segments.  The tiled entry
+ * points return NAMSA_ERR_UNSUPPORTED at other sizes.  No counterpart in the
+ * reference (its cuFFT plans own their layouts); for per-pass timing and
+ * custom slice loops at those sizes.
+ */
+int namsa_wave_tiled(int Ny, int Nx);
+int namsa_slice_rows_tiled(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo,
+                           int band_hi, void* stream);
+int namsa_slice_cols_tiled(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
+                           void* stream);
 
 /*
  * Batched multislice, fused (replaces MSAGPU.__propagate_beams,

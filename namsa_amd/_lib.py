@@ -13,7 +13,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().with_name("libnamsa_hip.so")
 HEADER_PATH = Path(__file__).resolve().parent.parent / "include" / "namsa_hip.h"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OUT_CBED = 0
 OUT_INCIDENT = 1
@@ -53,6 +53,9 @@ SIGNATURES = {
     "namsa_probe_init": (_i, [_vp, _vp, _i, _i, _i, _f, _vp, _vp]),
     "namsa_slice_rows": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
     "namsa_slice_cols": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
+    "namsa_wave_tiled": (_i, [_i, _i]),
+    "namsa_slice_rows_tiled": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
+    "namsa_slice_cols_tiled": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
     "namsa_build_potential": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _vp]),
     "namsa_gemm_bf16": (_i, [_vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _i, _i, _i, _i, _i,
                              _f, _f, _vp, _i, _i, _vp, _vp]),

@@ -126,12 +126,14 @@ def tune(n: int, batch: int = 0, reps: int = 20, write_cache: bool = True, verbo
     pm = t[0].flip(0).contiguous()
     st = stream_handle()
     lo, hi = band if band is not None else default_band(n)
-    result = {"n": n, "batch": batch, "band": [lo, hi], "row": {}, "col": {}}
+    # the wave layout namsa_multislice's slice loop uses at this size
+    sfx = "_tiled" if _lib.load().namsa_wave_tiled(n, n) else ""
+    result = {"n": n, "batch": batch, "band": [lo, hi], "row": {}, "col": {}, "tiled": bool(sfx)}
     i = [0]
 
     def step():
-        call("namsa_slice_rows", ptr(psi), batch, n, n, ptr(t[1 + i[0] % 3]), lo, hi, st)
-        call("namsa_slice_cols", ptr(psi), batch, n, n, ptr(pm), lo, hi, st)
+        call("namsa_slice_rows" + sfx, ptr(psi), batch, n, n, ptr(t[1 + i[0] % 3]), lo, hi, st)
+        call("namsa_slice_cols" + sfx, ptr(psi), batch, n, n, ptr(pm), lo, hi, st)
         i[0] += 1
 
     for pass_, key in ((PASS_COL, "col"), (PASS_ROW, "row")):

@@ -310,29 +310,65 @@ int namsa_probe_init(const void* psi_k, const float* pos_yx, int B, int Ny, int 
   return col_pass(Ny, COL_PROBE, a, as_stream(stream));
 }
 
-int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo,
-                     int band_hi, void* stream) {
-  if (int rc = check_grid(Ny, Nx, "namsa_slice_rows")) return rc;
-  if (B <= 0 || !psi_ws || !t_slice) return fail(kErrArg, "namsa_slice_rows: bad arguments");
-  if (int rc = check_band(Nx, band_lo, band_hi, "namsa_slice_rows")) return rc;
+}  // extern "C"
+
+namespace {
+
+int slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo, int band_hi, int tiled,
+               void* stream, const char* who) {
+  if (int rc = check_grid(Ny, Nx, who)) return rc;
+  if (B <= 0 || !psi_ws || !t_slice) return fail(kErrArg, "%s: bad arguments", who);
+  if (int rc = check_band(Nx, band_lo, band_hi, who)) return rc;
+  if (tiled && !(tiled_layout(Ny) && tiled_layout(Nx)))
+    return fail(kErrUnsupported, "%s: no tiled layout at %d x %d", who, Ny, Nx);
   RowArgs r = row_args(static_cast<float2*>(psi_ws), B, Ny, Nx, static_cast<const float2*>(t_slice), 1.0f,
                        band_lo, band_hi);
   r.store_lo = band_lo / kStoreAlign * kStoreAlign;
   r.store_hi = std::min(Nx, (band_hi + kStoreAlign - 1) / kStoreAlign * kStoreAlign);
+  r.tiled = tiled;
   return row_pass(Nx, ROW_STEP, r, as_stream(stream));
 }
 
-int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
-                     void* stream) {
-  if (int rc = check_grid(Ny, Nx, "namsa_slice_cols")) return rc;
-  if (B <= 0 || !psi_ws || !pm) return fail(kErrArg, "namsa_slice_cols: bad arguments");
-  if (int rc = check_band(Nx, band_lo, band_hi, "namsa_slice_cols")) return rc;
+int slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi, int tiled,
+               void* stream, const char* who) {
+  if (int rc = check_grid(Ny, Nx, who)) return rc;
+  if (B <= 0 || !psi_ws || !pm) return fail(kErrArg, "%s: bad arguments", who);
+  if (int rc = check_band(Nx, band_lo, band_hi, who)) return rc;
+  if (tiled && !(tiled_layout(Ny) && tiled_layout(Nx)))
+    return fail(kErrUnsupported, "%s: no tiled layout at %d x %d", who, Ny, Nx);
   ColArgs a = col_args(static_cast<float2*>(psi_ws), B, Nx);
   a.pm = static_cast<const float2*>(pm);
   a.pm_scale = 1.0f / static_cast<float>(static_cast<long long>(Ny) * Nx);
   a.band_lo = band_lo;
   a.band_hi = band_hi;
+  a.tiled = tiled;
   return col_pass(Ny, COL_PROP, a, as_stream(stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int namsa_slice_rows(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo, int band_hi,
+                     void* stream) {
+  return slice_rows(psi_ws, B, Ny, Nx, t_slice, band_lo, band_hi, 0, stream, "namsa_slice_rows");
+}
+
+int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
+                     void* stream) {
+  return slice_cols(psi_ws, B, Ny, Nx, pm, band_lo, band_hi, 0, stream, "namsa_slice_cols");
+}
+
+int namsa_wave_tiled(int Ny, int Nx) { return tiled_layout(Ny) && tiled_layout(Nx) ? 1 : 0; }
+
+int namsa_slice_rows_tiled(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo, int band_hi,
+                           void* stream) {
+  return slice_rows(psi_ws, B, Ny, Nx, t_slice, band_lo, band_hi, 1, stream, "namsa_slice_rows_tiled");
+}
+
+int namsa_slice_cols_tiled(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
+                           void* stream) {
+  return slice_cols(psi_ws, B, Ny, Nx, pm, band_lo, band_hi, 1, stream, "namsa_slice_cols_tiled");
 }
 
 }  // extern "C"
@@ -363,6 +399,9 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
   const long long plane = static_cast<long long>(Ny) * Nx;
   const float inv_n2 = 1.0f / static_cast<float>(plane);
   const int G = (group > 0 && group < B) ? group : B;
+  // In CBED mode the wave never leaves the library, so where it pays the
+  // slice loop keeps it in the tiled layout (passes.hpp).
+  const bool tiled = mode == NAMSA_OUT_CBED && tiled_layout(Nx) && tiled_layout(Ny);
 
   for (int g0 = 0; g0 < B; g0 += G) {
     const int Bg = std::min(G, B - g0);
@@ -378,6 +417,7 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
     a.pm_scale = inv_n2;
     a.band_lo = band_lo;
     a.band_hi = band_hi;
+    a.tiled = tiled ? 1 : 0;
     if (int rc = col_pass(Ny, COL_PROBE, a, st)) return rc;
 
     if (mode == NAMSA_OUT_INCIDENT) {
@@ -400,6 +440,7 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
       RowArgs r = row_args(psi, Bg, Ny, Nx, tt + i * plane, 1.0f, lo, hi);
       r.store_lo = store_lo;
       r.store_hi = store_hi;
+      r.tiled = a.tiled;
       if (int rc = row_pass(Nx, ROW_STEP, r, st, precise)) return rc;
       if (i < last_prop)
         if (int rc = col_pass(Ny, COL_PROP, a, st, precise)) return rc;

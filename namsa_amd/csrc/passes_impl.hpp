@@ -129,19 +129,36 @@ __device__ __forceinline__ float2 unit_phase(double turns) {
 }
 
 // ------------------------------------------------------------------ row pass
-template <class S, int OP>
+// TL: the wave is stored in 2 x 4 tiles (tile_offset); rows r = 2k, 2k + 1 of
+// the stacked images form tile row k, and element t + T m of a row sits at a
+// lane base plus 2 T m.  The two rows of a workgroup are interleaved over the
+// lanes (row g = lane % 2), so that each wavefront access covers whole
+// tiles: with device-scope stores, half-tile pieces written by two
+// wavefronts took 40% longer than whole tiles (2048^2, 6 images: a
+// memory-only row pass 69.5 against 49.2 us).  Skewing the second row's LDS
+// buffer against bank conflicts between the rows made no difference.
+template <class S, bool TL>
+struct RowMap {
+  static constexpr int T = S::N / S::E, G = kRowThreads / T;
+  static_assert(!TL || G == 2, "tiled rows pair two rows per workgroup");
+  __device__ static int g() { return TL ? threadIdx.x % G : threadIdx.x / T; }
+  __device__ static int t() { return TL ? threadIdx.x / G : threadIdx.x % T; }
+};
+
+template <class S, int OP, bool TL = false>
 __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(RowArgs a) {
   constexpr int N = S::N, E = S::E;
   using F = Fft<N, E, typename S::RL, S::CT>;
   constexpr int T = F::T;
   constexpr int G = kRowThreads / T;
   constexpr bool STEP = OP == ROW_STEP;
-  using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+  using Sync = typename std::conditional<(T <= 64 && !TL), WaveSync, BlockSync>::type;
   using Lds = RowLdsT<S::LS>;
+  using Map = RowMap<S, TL>;
   __shared__ float2 smem[G * Lds::padded(N)];
 
-  const int g = threadIdx.x / T;
-  const int t = threadIdx.x % T;
+  const int g = Map::g();
+  const int t = Map::t();
   Lds lds{smem + g * Lds::padded(N)};
   float2 tw[F::NTW_ALLOC];
   F::twiddles(t, tw);
@@ -161,12 +178,15 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
     trow += row_step;
     if (trow >= rows_img) trow -= rows_img;
   };
+  constexpr int XS = TL ? 2 * T : T;  // element stride of a row in the wave layout
+  static_assert(!TL || T % 4 == 0, "tiled rows need whole tiles per thread group");
+  auto row_base = [&](long long r) { return TL ? a.psi + tile_offset(r, t, N) : a.psi + r * N + t; };
   auto load = [&](long long r, float2(&v)[E], float2(&tv)[E]) {
-    const float2* p = a.psi + r * N + t;
+    const float2* p = row_base(r);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int x = t + T * m;
-      v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + T * m) : make_float2(0.f, 0.f);
+      v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + XS * m) : make_float2(0.f, 0.f);
     }
     if constexpr (STEP) {
       const float2* tr = a.tsl + trow * N + t;
@@ -209,11 +229,11 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
       F::template run<false, Sync>(v, lds, t, tw);
     }
     if (row < a.n_rows) {
-      float2* p = a.psi + row * N + t;
+      float2* p = row_base(row);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int x = t + T * m;
-        if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + T * m, v[m]);
+        if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + XS * m, v[m]);
       }
     }
   }
@@ -224,18 +244,19 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
 // image_chunk consecutive images of the probe group, which it runs in turn
 // with the slice row held in registers, so a slice row is read once per
 // chunk instead of once per image.
-template <class S>
+template <class S, bool TL = false>
 __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_step_tr_kernel(RowArgs a) {
   constexpr int N = S::N, E = S::E;
   using F = Fft<N, E, typename S::RL, S::CT>;
   constexpr int T = F::T;
   constexpr int G = kRowThreads / T;
-  using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+  using Sync = typename std::conditional<(T <= 64 && !TL), WaveSync, BlockSync>::type;
   using Lds = RowLdsT<S::LS>;
+  using Map = RowMap<S, TL>;
   __shared__ float2 smem[G * Lds::padded(N)];
 
-  const int g = threadIdx.x / T;
-  const int t = threadIdx.x % T;
+  const int g = Map::g();
+  const int t = Map::t();
   Lds lds{smem + g * Lds::padded(N)};
   float2 tw[F::NTW_ALLOC];
   F::twiddles(t, tw);
@@ -255,12 +276,15 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_step_tr_
 #pragma unroll
     for (int m = 0; m < E; ++m) tv[m] = tr[T * m];
     for (int b = b0; b < b1; ++b) {
-      float2* p = a.psi + (static_cast<long long>(b) * rows_img + y) * N + t;
+      const long long r = static_cast<long long>(b) * rows_img + y;
+      constexpr int XS = TL ? 2 * T : T;  // element stride of a row in the wave layout
+      static_assert(!TL || T % 4 == 0, "tiled rows need whole tiles per thread group");
+      float2* p = TL ? a.psi + tile_offset(r, t, N) : a.psi + r * N + t;
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int x = t + T * m;
-        v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + T * m) : make_float2(0.f, 0.f);
+        v[m] = (x >= a.band_lo && x < a.band_hi) ? ld<S::NT>(p + XS * m) : make_float2(0.f, 0.f);
       }
       F::template run<true, Sync>(v, lds, t, tw);
 #pragma unroll
@@ -270,7 +294,7 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_step_tr_
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int x = t + T * m;
-          if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + T * m, v[m]);
+          if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + XS * m, v[m]);
         }
       }
     }
@@ -320,14 +344,21 @@ __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(
   const bool live = OP != COL_READOUT || !a.apply_pm ||
                     (strip * C < a.band_hi && strip * C + C > a.band_lo);
 
-  // SB addressing: element (jt + T m, col) of a plane is at byte lane_off + m * row_block
-  // (the launcher checks that a plane's bytes fit 32 bits)
-  const unsigned lane_off = static_cast<unsigned>((jt * stride + col) * static_cast<long long>(sizeof(float2)));
-  const unsigned row_block = static_cast<unsigned>(T * stride * static_cast<long long>(sizeof(float2)));
+  // Element (jt + T m, col) of a row-major plane is rm_lane + m T Nx; of the
+  // wave in 2 x 4 tiles (a.tiled, T even) st_lane + m T Nx.  SB addressing:
+  // the byte offsets lane_off (wave) and pm_off (row-major) plus m * row_block
+  // (the launcher checks that a plane's bytes fit 32 bits).
+  static_assert(T % 2 == 0, "tiled columns need whole tile rows per element");
+  const long long rm_lane = jt * stride + col;
+  const long long st_lane = a.tiled ? tile_offset(jt, col, stride) : rm_lane;
+  const long long m_stride = T * stride;
+  const unsigned lane_off = static_cast<unsigned>(st_lane * static_cast<long long>(sizeof(float2)));
+  const unsigned pm_off = static_cast<unsigned>(rm_lane * static_cast<long long>(sizeof(float2)));
+  const unsigned row_block = static_cast<unsigned>(m_stride * static_cast<long long>(sizeof(float2)));
   const unsigned plane_bytes = static_cast<unsigned>(plane * static_cast<long long>(sizeof(float2)));
   auto pm_at = [&](int m) -> float2 {
-    if constexpr (S::SB) return buf_ld<false>(plane_rsrc(a.pm, plane_bytes), lane_off, m * row_block);
-    else return a.pm[(jt + T * m) * stride + col];
+    if constexpr (S::SB) return buf_ld<false>(plane_rsrc(a.pm, plane_bytes), pm_off, m * row_block);
+    else return a.pm[rm_lane + m * m_stride];
   };
 
   constexpr bool PM_AT_USE = OP == COL_PROP && S::PG;
@@ -354,9 +385,9 @@ __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = buf_ld<S::NT>(img, lane_off, m * row_block);
     } else {
-      const float2* p = a.psi + b * plane + col;
+      const float2* p = a.psi + b * plane + st_lane;
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = ld<S::NT>(p + (jt + T * m) * stride);
+      for (int m = 0; m < E; ++m) v[m] = ld<S::NT>(p + m * m_stride);
     }
   };
 
@@ -366,7 +397,7 @@ __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(
     if (b < a.B && live) load(b, pv);
   }
   for (; b < a.B; b += step) {
-    float2* base = a.psi + b * plane + col;
+    float2* base = a.psi + b * plane + st_lane;
     float2 v[E];
 
     if constexpr (OP == COL_PROBE) {
@@ -460,7 +491,7 @@ __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(
       for (int m = 0; m < E; ++m) buf_st<S::NT>(img, lane_off, m * row_block, v[m]);
     } else {
 #pragma unroll
-      for (int m = 0; m < E; ++m) st<S::NT>(base + (jt + T * m) * stride, v[m]);
+      for (int m = 0; m < E; ++m) st<S::NT>(base + m * m_stride, v[m]);
     }
   }
 }
@@ -475,11 +506,15 @@ int persistent_cap(K kernel, int threads) {
   return per_cu * device_cu_count();
 }
 
-template <class S, int OP>
+template <class S, int OP, bool TL = false>
 int launch_row_op(const RowArgs& a, hipStream_t st) {
+  if constexpr (!TL && OP == ROW_STEP && tiled_layout(S::N)) {
+    if (a.tiled) return launch_row_op<S, OP, true>(a, st);
+  }
+  if (!TL && a.tiled) return fail(kErrArg, "row pass: no tiled-layout form of this op at N = %d", S::N);
   if constexpr (S::TR && OP == ROW_STEP) {
     // image chunks sized so that there are about two tasks per resident workgroup
-    auto k = row_step_tr_kernel<S>;
+    auto k = row_step_tr_kernel<S, TL>;
     constexpr int G = kRowThreads / (S::N / S::E);
     static const int cap = persistent_cap(k, kRowThreads);
     const int groups = (a.rows_per_image + G - 1) / G;
@@ -492,7 +527,7 @@ int launch_row_op(const RowArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k, dim3(grid), dim3(kRowThreads), 0, st, c);
     return check_hip(hipGetLastError(), "row pass launch");
   }
-  auto k = row_kernel<S, OP>;
+  auto k = row_kernel<S, OP, TL>;
   constexpr int G = kRowThreads / (S::N / S::E);
   static const int cap = persistent_cap(k, kRowThreads);
   const long long blocks = (a.n_rows + G - 1) / G;

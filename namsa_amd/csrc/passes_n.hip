@@ -175,7 +175,7 @@ using C2 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false>;
 using R2 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true>;
 using R3 = RowS<2048, 16, Radices<16, 16, 8>, false, false, false, 0, true>;
 using R4 = RowS<2048, 16, Radices<16, 16, 8>, false, false, true, 0, true>;
-using R5 = RowS<2048, 16, Radices<16, 16, 8>, false, false, false, 0, true, true>;
+using R5 = RowS<2048, 16, Radices<16, 16, 8>, false, false, false, 3, true, true>;
 using C3 = ColS<2048, 16, Radices<16, 16, 8>, 4, true, false, false, true>;
 using C4 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, false, 4, true, true>;
 using C5 = ColS<2048, 16, Radices<16, 16, 8>, 4, false, false, false, true, 4, true, true>;

@@ -51,6 +51,11 @@ def test_argument_errors_report_without_gpu():
     assert rc == -1 and b"band" in lib.namsa_last_error()
     with pytest.raises(L.NamsaError):
         L.call("namsa_norm2", None, 0, None, None)
+    assert lib.namsa_wave_tiled(2048, 2048) == 1 and lib.namsa_wave_tiled(512, 512) == 0
+    rc = lib.namsa_slice_rows_tiled(ctypes.c_void_p(16), 1, 512, 512, ctypes.c_void_p(16), 0, 512, None)
+    assert rc == -2 and b"tiled" in lib.namsa_last_error()
+    rc = lib.namsa_slice_cols_tiled(ctypes.c_void_p(16), 1, 512, 512, ctypes.c_void_p(16), 0, 512, None)
+    assert rc == -2 and b"tiled" in lib.namsa_last_error()
     # sized for the narrowest column tile of any schedule (4 columns)
     assert lib.namsa_det_workspace_doubles(4, 512, 512, 2) == 4 * (512 // 4) * 2
     assert lib.namsa_det_workspace_doubles(4, 500, 512, 2) == 0

@@ -95,3 +95,67 @@ def test_lds_root_twiddles_match_register_twiddles(gpu, N, reg, lds):
     d = rel_l2(b, a)
     print(f"   register vs LDS-root twiddles: relL2 {d:.2e}; vs oracle {rel_l2(a, exit_ref):.2e} {rel_l2(b, exit_ref):.2e}")
     assert d < 1e-6
+
+
+def _tile(a):
+    """[B][Ny][Nx] -> the same planes in 2 x 4 tiles: element (y, x) at
+    ((y//2)(Nx//4) + x//4)*8 + (y%2)*4 + x%4 (namsa_slice_rows_tiled)."""
+    B, Ny, Nx = a.shape
+    return np.ascontiguousarray(a.reshape(B, Ny // 2, 2, Nx // 4, 4).transpose(0, 1, 3, 2, 4)).reshape(B, Ny, Nx)
+
+
+def _untile(a):
+    B, Ny, Nx = a.shape
+    return np.ascontiguousarray(a.reshape(B, Ny // 2, Nx // 4, 2, 4).transpose(0, 1, 3, 2, 4)).reshape(B, Ny, Nx)
+
+
+def test_tile_helpers_restate_the_offset():
+    Ny, Nx = 6, 12
+    a = np.arange(Ny * Nx).reshape(1, Ny, Nx)
+    flat = _tile(a).reshape(-1)
+    for y in range(Ny):
+        for x in range(Nx):
+            assert flat[(y // 2) * 2 * Nx + (x // 4) * 8 + (y % 2) * 4 + x % 4] == a[0, y, x]
+    assert np.array_equal(_untile(_tile(a)), a)
+
+
+@pytest.mark.parametrize("precise", [False, True])
+def test_tiled_slice_passes_match_row_major(gpu, precise):
+    """The slice step on the 2 x 4 tiled wave (the layout namsa_multislice's
+    CBED loop keeps at 2048^2) computes the same float operations as on the
+    row-major wave, only at other addresses: bit-identical after untiling,
+    for the default schedules and their thick-run partners."""
+    import torch
+
+    N, B = 2048, 2
+    lib = L.load()
+    assert lib.namsa_wave_tiled(N, N) == 1
+    rng = np.random.default_rng(3)
+    psi = (rng.standard_normal((B, N, N)) + 1j * rng.standard_normal((B, N, N))).astype(np.complex64)
+    t = np.exp(1j * rng.uniform(0, 2 * np.pi, (N, N))).astype(np.complex64)
+    pm = (np.exp(1j * rng.uniform(0, 2 * np.pi, (N, N))) * (rng.random((N, N)) < 0.9)).astype(np.complex64)
+    lo, hi = autotune.default_band(N)
+    pm[:, :lo] = 0
+    pm[:, hi:] = 0
+    keep = (autotune.selected(N, 0), autotune.selected(N, 1))
+    try:
+        if precise:
+            autotune.select(N, 0, lib.namsa_sched_precise(N, 0, keep[0]))
+            autotune.select(N, 1, lib.namsa_sched_precise(N, 1, keep[1]))
+        names = (autotune.schedules(N, 0)[autotune.selected(N, 0)], autotune.schedules(N, 1)[autotune.selected(N, 1)])
+        print(f"N={N} B={B} schedules {names}", flush=True)
+        td, pmd = torch.from_numpy(t).to(gpu), torch.from_numpy(pm).to(gpu)
+        st = L.stream_handle()
+        out = []
+        for sfx, a in (("", psi), ("_tiled", _tile(psi))):
+            w = torch.from_numpy(a).to(gpu)
+            for band in ((0, N), (lo, hi)):
+                L.call("namsa_slice_rows" + sfx, L.ptr(w), B, N, N, L.ptr(td), band[0], band[1], st)
+                L.call("namsa_slice_cols" + sfx, L.ptr(w), B, N, N, L.ptr(pmd), lo, hi, st)
+            out.append(w.cpu().numpy())
+    finally:
+        autotune.select(N, 0, keep[0])
+        autotune.select(N, 1, keep[1])
+    rm, tl = out[0], _untile(out[1])
+    assert np.isfinite(rm).all()
+    assert np.array_equal(rm.view(np.uint64), tl.view(np.uint64))
