@@ -86,6 +86,24 @@ def setup(edge):
     return sim, dets
 
 
+def copy_calibration():
+    """Two 1 GiB device-to-device copies (the second timed): the reference
+    dispatches that scale the FETCH_SIZE / WRITE_SIZE counters to bytes.
+    Returns the copy's read + write rate in GB/s."""
+    import torch
+
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    b.copy_(a)
+    c1.record()
+    torch.cuda.synchronize()
+    return 2 * a.numel() * 4 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+
+
 def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
     """Live per-kernel timing of the two slice-step passes on one group of G
     probes, on the stream the kernels run on: R back-to-back launches of each
@@ -743,6 +761,8 @@ def bench_config5(args):
     from namsa_amd import autotune
     from namsa_amd.msa import cache_group
 
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     rec, info = structures.polycrystal(seed=5)
     log(f"config5: polycrystal {info['atoms']} atoms, grains {info['grains']} ({time.perf_counter() - t0:.1f} s)")
@@ -805,6 +825,9 @@ def bench_config5(args):
     rf = pass_roofline(sim, pos, ws, group, lo, hi, sched["col"], args.kernel_reps)
     autotune.select(Nx, 0, keep[0])
     autotune.select(Nx, 1, keep[1])
+    tf = ROOT / "profiles" / "hbm_traffic_config5_2048.json"
+    if tf.exists():
+        rf["traffic"] = json.loads(tf.read_text()).get(rf["kernel"])
     out = {
         "metric": METRIC, "value": value, "unit": "probe-positions·slices/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
@@ -936,20 +959,9 @@ def main():
     if args.workload == "stream":
         return bench_stream(args)
 
-    copy_gbs = None
-    if args.copy_calibration:
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        b.copy_(a)
-        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        c0.record()
-        b.copy_(a)
-        c1.record()
-        torch.cuda.synchronize()
-        copy_gbs = 2 * a.numel() * 4 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+    copy_gbs = copy_calibration() if args.copy_calibration else None
+    if copy_gbs is not None:
         log(f"copy calibration: 1 GiB read + 1 GiB write, {copy_gbs:.0f} GB/s")
-        del a, b
 
     log("setup: structure, tables, slices, probe, raster")
     sim, dets = setup(args.probes)
