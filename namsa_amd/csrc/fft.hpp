@@ -139,6 +139,45 @@ struct Dft<16, INV> {
   }
 };
 
+// Radix 6 and 12 as prime-factor (Good-Thomas) transforms: for R = R1 R2 with
+// gcd(R1, R2) = 1, input n = (R2 n1 + R1 n2) mod R and output
+// k = (R2 (R2^-1 mod R1) k1 + R1 (R1^-1 mod R2) k2) mod R turn the length-R
+// DFT into R2 DFTs of length R1 followed by R1 of length R2 with no twiddles
+// between them (W_R^(nk) = W_R1^(n1 k1) W_R2^(n2 k2)), so the only constants
+// are the radix-3 pair's.
+//   R = 6 = 2 x 3:  n = (3 n1 + 2 n2) mod 6,  k = (3 k1 + 4 k2) mod 6
+//   R = 12 = 3 x 4: n = (4 n1 + 3 n2) mod 12, k = (4 k1 + 9 k2) mod 12
+template <int R1, int R2, int M1, int M2, bool INV>
+__device__ __forceinline__ void pfa(float2* a) {
+  constexpr int R = R1 * R2;
+  float2 y[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) y[n2][n1] = a[(R2 * n1 + R1 * n2) % R];
+    Dft<R1, INV>::run(y[n2]);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    float2 z[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) z[n2] = y[n2][k1];
+    Dft<R2, INV>::run(z);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) a[(M1 * k1 + M2 * k2) % R] = z[k2];
+  }
+}
+
+template <bool INV>
+struct Dft<6, INV> {
+  static __device__ __forceinline__ void run(float2* a) { pfa<2, 3, 3, 4, INV>(a); }
+};
+
+template <bool INV>
+struct Dft<12, INV> {
+  static __device__ __forceinline__ void run(float2* a) { pfa<3, 4, 4, 9, INV>(a); }
+};
+
 // ------------------------------------------------------------------ schedules
 template <int... Rs>
 struct Radices {};
@@ -174,15 +213,15 @@ template <int E, int NS>
 struct TwCountC<E, NS> {
   static constexpr int value = 0;
 };
-// registers per butterfly in compact mode: for R >= 8 the powers of two w,
-// w^2, .. w^(R/2) (log2 R pairs), the others products of at most three of
-// them; for R <= 4 only w (w^2, w^3 by a two-multiply chain).  A
-// forward and an inverse transform multiply to |w^r|^2 per twiddle, so the
-// magnitude error of the formed powers is what accumulates over the slices:
-// with at most two products per power it stays ~1.5x that of directly
-// rounded twiddles (a 1-pair power chain is ~3x and grows to 1.6e-4 of the
-// CBED after 1000 slices).
-constexpr int compact_per_bfly(int R) { return R >= 16 ? 4 : (R >= 8 ? 3 : 1); }
+// registers per butterfly in compact mode: for R > 4 the powers of two w,
+// w^2, .. up to w^(R-1) (w, w^2, w^4 for R = 6, 8; and w^8 for R = 12, 16),
+// the others products of at most three of them; for R <= 4 only w (w^2, w^3
+// by a two-multiply chain).  A forward and an inverse transform multiply to
+// |w^r|^2 per twiddle, so the magnitude error of the formed powers is what
+// accumulates over the slices: with at most two products per power it stays
+// ~1.5x that of directly rounded twiddles (a 1-pair power chain is ~3x and
+// grows to 1.6e-4 of the CBED after 1000 slices).
+constexpr int compact_per_bfly(int R) { return R > 8 ? 4 : (R > 4 ? 3 : 1); }
 template <int E, int NS, int R, int... Rs>
 struct TwCountC<E, NS, R, Rs...> {
   static constexpr int value =
@@ -375,7 +414,7 @@ struct Fft<N, E, Radices<Rs...>, CT, LR> {
   static constexpr int per_bfly() { return CT ? compact_per_bfly(R) : R - 1; }
   // exponent of the r-th stored twiddle of a butterfly
   template <int R>
-  static constexpr int stored_power(int r) { return CT && R >= 8 ? 1 << (r - 1) : r; }
+  static constexpr int stored_power(int r) { return CT && R > 4 ? 1 << (r - 1) : r; }
   static constexpr int NTW_TAB = NTW > 0 ? NTW : 1;
   static constexpr int NTW_ALLOC = LR ? 1 : NTW_TAB;  // twiddle registers a caller holds
   static constexpr int NSTAGES = sizeof...(Rs);
@@ -435,9 +474,8 @@ struct Fft<N, E, Radices<Rs...>, CT, LR> {
 #pragma unroll
       for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
       if constexpr (NS > 1) {
-        if constexpr (CT && R >= 8) {
+        if constexpr (CT && R > 4) {
           // stored w^(2^j); w^r = w^h w^(r-h), h the top bit of r
-          static_assert((R & (R - 1)) == 0, "compact twiddles need a power-of-two radix");
           constexpr int P = compact_per_bfly(R);
           float2 pw[R];
 #pragma unroll

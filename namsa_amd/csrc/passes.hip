@@ -45,15 +45,16 @@ static const SchedTable* table_for(int n) {
 // addressing) at 128^2 and 384^2-2048^2, 4-24% faster per slice step than the
 // persistent one-strip-per-workgroup schedules, which stay the defaults at
 // 192^2 and 256^2.  Their compact forms keep the strip's propagator in
-// registers for several images (.i<k>) at 768^2-2048^2; the forms holding
-// every twiddle read it at each use instead (.pg), and at 768^2 thick runs
-// take the double-buffered persistent schedule, faster there than either.
-// At 2048^2 the row step holds each transmission row across the group's
-// images (.tr; 5-8% faster per slice step there, slower at smaller sizes,
-// where the slice rows stay cached).
+// registers for several images (.i<k>) at 1024^2-2048^2; the forms holding
+// every twiddle read it at each use instead (.pg).  At 192^2, 384^2 and
+// 768^2 the schedules whose first stage is a radix-12 prime-factor
+// butterfly (three or four stages instead of four or five) run a slice step
+// 11-23% faster than the radix-3/4 ones.  At 2048^2 the row step holds each
+// transmission row across the group's images (.tr; 5-8% faster per slice
+// step there, slower at smaller sizes, where the slice rows stay cached).
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 1,  0,  0,  2,  6,  2,  3,  4};
-static const int kDefaultCol[kNumSizes] = { 2,  1,  2,  5, 15,  6,  6,  6};
+static const int kDefaultRow[kNumSizes] = { 1,  2,  0,  4,  6,  4,  3,  4};
+static const int kDefaultCol[kNumSizes] = { 2,  2,  2,  7, 15,  9,  6,  6};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];

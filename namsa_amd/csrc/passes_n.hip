@@ -30,9 +30,18 @@ using R0 = RowS<192, 12, Radices<3, 4, 4, 4>, false, false>;
 using R1 = RowS<192, 12, Radices<3, 4, 4, 4>, true, false>;
 using C0 = ColS<192, 12, Radices<3, 4, 4, 4>, 16, false, false>;
 using C1 = ColS<192, 12, Radices<3, 4, 4, 4>, 8, true, false>;
-static const RowSchedEntry kRows[] = {{"e12.r3444", launch_row<R0, R0>}, {"e12.r3444.pf", launch_row<R1, R0>}};
+using R2 = RowS<192, 12, Radices<12, 4, 4>, false, false>;
+using R3 = RowS<192, 12, Radices<12, 4, 4>, true, false>;
+using C2 = ColS<192, 12, Radices<12, 4, 4>, 16, false, false>;
+using C3 = ColS<192, 12, Radices<12, 4, 4>, 8, true, false>;
+static const RowSchedEntry kRows[] = {{"e12.r3444", launch_row<R0, R0>},
+                                      {"e12.r3444.pf", launch_row<R1, R0>},
+                                      {"e12.r12.4.4", launch_row<R2, R0>},
+                                      {"e12.r12.4.4.pf", launch_row<R3, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r3444.c16", launch_col<C0, C0>, 16},
-                                      {"e12.r3444.c8.pf", launch_col<C1, C0>, 8}};
+                                      {"e12.r3444.c8.pf", launch_col<C1, C0>, 8},
+                                      {"e12.r12.4.4.c16", launch_col<C2, C0>, 16},
+                                      {"e12.r12.4.4.c8.pf", launch_col<C3, C0>, 8}};
 #elif NAMSA_PASS_N == 256
 using R0 = RowS<256, 8, Radices<8, 8, 4>, false, false>;
 using R1 = RowS<256, 8, Radices<8, 8, 4>, true, false>;
@@ -58,15 +67,31 @@ using C2 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, false, true>;
 using C3 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, true, false, true>;
 using C4 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
 using C5 = ColS<384, 12, Radices<3, 2, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
+using R3 = RowS<384, 12, Radices<12, 4, 4, 2>, false, false>;
+using R4 = RowS<384, 12, Radices<12, 4, 4, 2>, false, false, true>;
+using R5 = RowS<384, 12, Radices<6, 4, 4, 4>, false, false>;
+using R6 = RowS<384, 12, Radices<6, 4, 4, 4>, false, false, true>;
+using C6 = ColS<384, 12, Radices<12, 4, 4, 2>, 8, false, false, false, false, 4, true, true>;
+using C7 = ColS<384, 12, Radices<12, 4, 4, 2>, 8, false, false, false, true, 4, true, true>;
+using C8 = ColS<384, 12, Radices<6, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
+using C9 = ColS<384, 12, Radices<6, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
 static const RowSchedEntry kRows[] = {{"e12.r32444", launch_row<R0, R0>},
                                       {"e12.r32444.pf", launch_row<R1, R0>},
-                                      {"e12.r32444.ct", launch_row<R2, R0>}};
+                                      {"e12.r32444.ct", launch_row<R2, R0>},
+                                      {"e12.r12.4.4.2", launch_row<R3, R0>},
+                                      {"e12.r12.4.4.2.ct", launch_row<R4, R0>},
+                                      {"e12.r6444", launch_row<R5, R0>},
+                                      {"e12.r6444.ct", launch_row<R6, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r32444.c16", launch_col<C0, C0>, 16},
                                       {"e12.r32444.c8.pf", launch_col<C1, C0>, 8},
                                       {"e12.r32444.c8.pf.ct", launch_col<C2, C0>, 8},
                                       {"e12.r32444.c8.db.pf", launch_col<C3, C0>, 8},
                                       {"e12.r32444.c8.pg.sb.w4", launch_col<C4, C0>, 8},
-                                      {"e12.r32444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8}};
+                                      {"e12.r32444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8},
+                                      {"e12.r12.4.4.2.c8.pg.sb.w4", launch_col<C6, C0>, 8},
+                                      {"e12.r12.4.4.2.c8.pg.sb.w4.ct", launch_col<C7, C0>, 8},
+                                      {"e12.r6444.c8.pg.sb.w4", launch_col<C8, C0>, 8},
+                                      {"e12.r6444.c8.pg.sb.w4.ct", launch_col<C9, C0>, 8}};
 #elif NAMSA_PASS_N == 512
 using R0 = RowS<512, 8, Radices<8, 8, 8>, false, false>;
 using R1 = RowS<512, 8, Radices<8, 8, 8>, true, false>;
@@ -129,16 +154,28 @@ using C3 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, true, false, true>;
 using C4 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
 using C5 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
 using C6 = ColS<768, 12, Radices<3, 4, 4, 4, 4>, 8, false, false, false, true, 4, false, true, 2>;
+using R3 = RowS<768, 12, Radices<12, 4, 4, 4>, false, false>;
+using R4 = RowS<768, 12, Radices<12, 4, 4, 4>, false, false, true>;
+using C7 = ColS<768, 12, Radices<12, 4, 4, 4>, 8, false, false, false, true, 4, false, true, 2>;
+using C8 = ColS<768, 12, Radices<12, 4, 4, 4>, 8, true, false, true>;
+using C9 = ColS<768, 12, Radices<12, 4, 4, 4>, 8, false, false, false, true, 4, true, true>;
+using C10 = ColS<768, 12, Radices<12, 4, 4, 4>, 8, false, false, false, false, 4, true, true>;
 static const RowSchedEntry kRows[] = {{"e12.r34444", launch_row<R0, R0>},
                                       {"e12.r34444.pf", launch_row<R1, R0>},
-                                      {"e12.r34444.ct", launch_row<R2, R0>}};
+                                      {"e12.r34444.ct", launch_row<R2, R0>},
+                                      {"e12.r12.4.4.4", launch_row<R3, R0>},
+                                      {"e12.r12.4.4.4.ct", launch_row<R4, R0>}};
 static const ColSchedEntry kCols[] = {{"e12.r34444.c8", launch_col<C0, C0>, 8},
                                       {"e12.r34444.c8.pf", launch_col<C1, C0>, 8},
                                       {"e12.r34444.c8.pf.ct", launch_col<C2, C0>, 8},
                                       {"e12.r34444.c8.db.pf", launch_col<C3, C0>, 8},
                                       {"e12.r34444.c8.pg.sb.w4", launch_col<C4, C0>, 8},
                                       {"e12.r34444.c8.pg.sb.w4.ct", launch_col<C5, C0>, 8, "e12.r34444.c8.db.pf"},
-                                      {"e12.r34444.c8.sb.w4.i2.ct", launch_col<C6, C0>, 8, "e12.r34444.c8.db.pf"}};
+                                      {"e12.r34444.c8.sb.w4.i2.ct", launch_col<C6, C0>, 8, "e12.r34444.c8.db.pf"},
+                                      {"e12.r12.4.4.4.c8.sb.w4.i2.ct", launch_col<C7, C0>, 8, "e12.r12.4.4.4.c8.db.pf"},
+                                      {"e12.r12.4.4.4.c8.db.pf", launch_col<C8, C0>, 8},
+                                      {"e12.r12.4.4.4.c8.pg.sb.w4.ct", launch_col<C9, C0>, 8},
+                                      {"e12.r12.4.4.4.c8.pg.sb.w4", launch_col<C10, C0>, 8}};
 #elif NAMSA_PASS_N == 1024
 using R0 = RowS<1024, 16, Radices<16, 8, 8>, false, false>;
 using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;

@@ -91,6 +91,7 @@ def test_precise_schedule_policy():
     named = {(768, 1, "e12.r34444.c8.pg.sb.w4.ct"): "e12.r34444.c8.db.pf",
              (768, 1, "e12.r34444.c8.sb.w4.i2.ct"): "e12.r34444.c8.db.pf",
              (512, 1, "e8.r888.c8.pg.sb.w6.ct"): "e8.r888.c8.lr.pg.sb.w6",
+             (768, 1, "e12.r12.4.4.4.c8.sb.w4.i2.ct"): "e12.r12.4.4.4.c8.db.pf",
              (1024, 1, "e16.r16.8.8.c8.sb.w4.i4.ct"): "e16.r16.8.8.c8.lr.pg.sb.w4",
              (2048, 0, "e16.r16.16.8.tr.ct"): "e16.r16.16.8.tr.ls",
              (2048, 1, "e16.r16.16.8.c4.sb.w4.i3.ct"): "e16.r16.16.8.c4.lr.pg.sb.w4"}
