@@ -122,9 +122,10 @@ def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
     # the wave layout namsa_multislice's CBED slice loop uses at this size
     # (the probe former's row-major state only seeds finite values)
     sfx = "_tiled" if L.load().namsa_wave_tiled(Ny, Nx) else ""
+    pm = sim.cbed_propagator()[0] if sfx else L.ptr(sim.pm_device)
     L.call("namsa_probe_init", L.ptr(sim.psi_k_device), L.ptr(pos), G, Ny, Nx, kmax, L.ptr(ws), st)
     L.call("namsa_slice_rows" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim._t[0]), 0, Nx, st)
-    L.call("namsa_slice_cols" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
+    L.call("namsa_slice_cols" + sfx, L.ptr(ws), G, Ny, Nx, pm, lo, hi, st)
     reps = max(2, reps)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     ev[0].record()
@@ -133,7 +134,7 @@ def pass_roofline(sim, pos, ws, G, lo, hi, col_sched, reps):
     ev[1].record()
     ev[2].record()
     for r in range(reps):
-        L.call("namsa_slice_cols" + sfx, L.ptr(ws), G, Ny, Nx, L.ptr(sim.pm_device), lo, hi, st)
+        L.call("namsa_slice_cols" + sfx, L.ptr(ws), G, Ny, Nx, pm, lo, hi, st)
     ev[3].record()
     torch.cuda.synchronize()
     row_ms = ev[0].elapsed_time(ev[1]) / reps
@@ -796,11 +797,13 @@ def bench_config5(args):
     kmax = np.float32(sim.kmax)
     st = L.stream_handle()
 
+    pm_arg, pm_flags = sim.cbed_propagator()
+
     def step():
         for c0 in range(0, P, chunk):
             B = min(chunk, P - c0)
             L.call("namsa_multislice", L.ptr(sim.psi_k_device), L.ptr(pos[c0]), B, Ny, Nx, kmax, L.ptr(sim._t), S,
-                   L.ptr(sim.pm_device), lo, hi, group, L.OUT_CBED, scale, L.ptr(ws), L.ptr(cbed), L.ptr(det),
+                   pm_arg, lo, hi, group, L.OUT_CBED | pm_flags, scale, L.ptr(ws), L.ptr(cbed), L.ptr(det),
                    3, L.ptr(det_ws), L.ptr(det_out[c0]), st)
 
     for _ in range(args.warmup):

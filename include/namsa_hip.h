@@ -35,6 +35,11 @@ extern "C" {
 #define NAMSA_OUT_CBED 0     /* transmit=True: float32 CBED intensities   */
 #define NAMSA_OUT_INCIDENT 1 /* transmit=False: complex incident probes   */
 #define NAMSA_OUT_EXIT 2     /* complex real-space exit waves              */
+/* Or-ed into namsa_multislice's mode with NAMSA_OUT_CBED where
+ * namsa_wave_tiled(Ny, Nx): pm is given in the 2 x 4 tile layout
+ * (namsa_tile_plane), which the tiled column pass then reads in 64-byte
+ * pieces too.  No counterpart in the reference. */
+#define NAMSA_PM_TILED 0x100
 
 int namsa_abi_version(void);
 const char* namsa_last_error(void);
@@ -207,7 +212,8 @@ int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int ba
 
 /*
  * The same two passes over a wave stored in 2 x 4 tiles: element (y, x) of a
- * [Ny][Nx] plane at ((y/2)(Nx/4) + x/4)*8 + (y%2)*4 + x%4.  In CBED mode
+ * [Ny][Nx] plane at ((y/2)(Nx/4) + x/4)*8 + (y%2)*4 + x%4 (the column pass
+ * takes pm in the same tiles; t_slice stays row-major).  In CBED mode
  * namsa_multislice keeps its internal wave in this layout at the sizes where
  * namsa_wave_tiled(Ny, Nx) returns 1 (2048 x 2048): the column pass then
  * reads 64-byte pieces instead of 32-byte row segments.  The tiled entry
@@ -220,6 +226,10 @@ int namsa_slice_rows_tiled(void* psi_ws, int B, int Ny, int Nx, const void* t_sl
                            int band_hi, void* stream);
 int namsa_slice_cols_tiled(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo, int band_hi,
                            void* stream);
+/* dst[tile_offset(y, x)] = src[y][x]: a [Ny][Nx] complex64 plane into the
+ * 2 x 4 tile layout (Ny even, Nx a multiple of 4; dst != src).  For the
+ * propagator of NAMSA_PM_TILED. */
+int namsa_tile_plane(const void* src, void* dst, int Ny, int Nx, void* stream);
 
 /*
  * Batched multislice, fused (replaces MSAGPU.__propagate_beams,

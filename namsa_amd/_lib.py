@@ -18,6 +18,7 @@ ABI_VERSION = 3
 OUT_CBED = 0
 OUT_INCIDENT = 1
 OUT_EXIT = 2
+PM_TILED = 0x100
 
 
 class NamsaError(RuntimeError):
@@ -56,6 +57,7 @@ SIGNATURES = {
     "namsa_wave_tiled": (_i, [_i, _i]),
     "namsa_slice_rows_tiled": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
     "namsa_slice_cols_tiled": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp]),
+    "namsa_tile_plane": (_i, [_vp, _vp, _i, _i, _vp]),
     "namsa_build_potential": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _vp]),
     "namsa_gemm_bf16": (_i, [_vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _vp, _ll, _ll, _ll, _i, _i, _i, _i, _i,
                              _f, _f, _vp, _i, _i, _vp, _vp]),

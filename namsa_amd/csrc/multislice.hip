@@ -72,6 +72,13 @@ __global__ void det_finalize_kernel(const double* __restrict__ ws, int tiles_per
   out[i] = s;
 }
 
+// dst[tile_offset(y, x)] = src[y][x], one element per thread (reads coalesced).
+__global__ void tile_plane_kernel(const float2* __restrict__ src, float2* __restrict__ dst, int Nx, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[tile_offset(i / Nx, i % Nx, Nx)] = src[i];
+}
+
 // ------------------------------------------------------------------ small kernels
 // psi_k and aperture (probe_kernels.cu:180-222): k in double on the linspace
 // grid, stored as float; chi in double with the reference's float constants.
@@ -342,6 +349,7 @@ int slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int band_lo,
   a.band_lo = band_lo;
   a.band_hi = band_hi;
   a.tiled = tiled;
+  a.pm_tiled = tiled;
   return col_pass(Ny, COL_PROP, a, as_stream(stream));
 }
 
@@ -361,6 +369,16 @@ int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int ba
 
 int namsa_wave_tiled(int Ny, int Nx) { return tiled_layout(Ny) && tiled_layout(Nx) ? 1 : 0; }
 
+int namsa_tile_plane(const void* src, void* dst, int Ny, int Nx, void* stream) {
+  if (!src || !dst || src == dst) return fail(kErrArg, "namsa_tile_plane: null or aliased planes");
+  if (Ny <= 0 || Nx <= 0 || Ny % 2 || Nx % 4)
+    return fail(kErrArg, "namsa_tile_plane: %d x %d is not whole 2 x 4 tiles", Ny, Nx);
+  const long long n = static_cast<long long>(Ny) * Nx;
+  hipLaunchKernelGGL(tile_plane_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), static_cast<const float2*>(src), static_cast<float2*>(dst), Nx, n);
+  return check_hip(hipGetLastError(), "tile_plane_kernel launch");
+}
+
 int namsa_slice_rows_tiled(void* psi_ws, int B, int Ny, int Nx, const void* t_slice, int band_lo, int band_hi,
                            void* stream) {
   return slice_rows(psi_ws, B, Ny, Nx, t_slice, band_lo, band_hi, 1, stream, "namsa_slice_rows_tiled");
@@ -379,6 +397,8 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
                     int S, const void* pm, int band_lo, int band_hi, int group, int mode, float out_scale,
                     void* psi_ws, float* cbed, const float* det, int nDet, double* det_ws, double* det_out,
                     double* mom_ws, double* mom_out, void* stream) {
+  const bool pm_tiled = (mode & NAMSA_PM_TILED) != 0;
+  mode &= ~NAMSA_PM_TILED;
   if (int rc = check_grid(Ny, Nx, "namsa_multislice")) return rc;
   if (B <= 0 || S < 0 || group < 0 || !psi_k || !pos_yx || !psi_ws)
     return fail(kErrArg, "namsa_multislice: B=%d S=%d group=%d or null probe/positions/workspace",
@@ -402,6 +422,9 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
   // In CBED mode the wave never leaves the library, so where it pays the
   // slice loop keeps it in the tiled layout (passes.hpp).
   const bool tiled = mode == NAMSA_OUT_CBED && tiled_layout(Nx) && tiled_layout(Ny);
+  if (pm_tiled && !tiled)
+    return fail(kErrArg, "namsa_multislice: NAMSA_PM_TILED needs NAMSA_OUT_CBED at a tiled size (not %d x %d)",
+                Ny, Nx);
 
   for (int g0 = 0; g0 < B; g0 += G) {
     const int Bg = std::min(G, B - g0);
@@ -418,6 +441,7 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
     a.band_lo = band_lo;
     a.band_hi = band_hi;
     a.tiled = tiled ? 1 : 0;
+    a.pm_tiled = pm_tiled ? 1 : 0;
     if (int rc = col_pass(Ny, COL_PROBE, a, st)) return rc;
 
     if (mode == NAMSA_OUT_INCIDENT) {

@@ -45,8 +45,9 @@ struct ColArgs {
   int n_det;
   double* det_ws;        // [B][strips_per_image][n_det] (READOUT)
   double* mom_ws;        // [B][strips_per_image][2]: sums of the CBED and of its square (READOUT), or null
-  int tiled;             // PROBE, PROP, READOUT: the wave is stored in 2 x 4 tiles (tile_offset); pm, psi_k,
+  int tiled;             // PROBE, PROP, READOUT: the wave is stored in 2 x 4 tiles (tile_offset); psi_k,
                          // cbed and det stay row-major
+  int pm_tiled;          // with tiled: pm is in the same tiles
 };
 
 // Tiled wave layout: element (y, x) of a [Ny][Nx] plane sits in the 64-byte

@@ -345,20 +345,22 @@ __global__ void __launch_bounds__(S::threads(), S::W > 0 ? S::W : 1) col_kernel(
                     (strip * C < a.band_hi && strip * C + C > a.band_lo);
 
   // Element (jt + T m, col) of a row-major plane is rm_lane + m T Nx; of the
-  // wave in 2 x 4 tiles (a.tiled, T even) st_lane + m T Nx.  SB addressing:
-  // the byte offsets lane_off (wave) and pm_off (row-major) plus m * row_block
-  // (the launcher checks that a plane's bytes fit 32 bits).
+  // wave in 2 x 4 tiles (a.tiled, T even) st_lane + m T Nx; pm is in either
+  // layout (a.pm_tiled).  SB addressing: the byte offsets lane_off (wave) and
+  // pm_off plus m * row_block (the launcher checks that a plane's bytes fit
+  // 32 bits).
   static_assert(T % 2 == 0, "tiled columns need whole tile rows per element");
   const long long rm_lane = jt * stride + col;
   const long long st_lane = a.tiled ? tile_offset(jt, col, stride) : rm_lane;
+  const long long pm_lane = a.tiled && a.pm_tiled ? st_lane : rm_lane;
   const long long m_stride = T * stride;
   const unsigned lane_off = static_cast<unsigned>(st_lane * static_cast<long long>(sizeof(float2)));
-  const unsigned pm_off = static_cast<unsigned>(rm_lane * static_cast<long long>(sizeof(float2)));
+  const unsigned pm_off = static_cast<unsigned>(pm_lane * static_cast<long long>(sizeof(float2)));
   const unsigned row_block = static_cast<unsigned>(m_stride * static_cast<long long>(sizeof(float2)));
   const unsigned plane_bytes = static_cast<unsigned>(plane * static_cast<long long>(sizeof(float2)));
   auto pm_at = [&](int m) -> float2 {
     if constexpr (S::SB) return buf_ld<false>(plane_rsrc(a.pm, plane_bytes), pm_off, m * row_block);
-    else return a.pm[rm_lane + m * m_stride];
+    else return a.pm[pm_lane + m * m_stride];
   };
 
   constexpr bool PM_AT_USE = OP == COL_PROP && S::PG;

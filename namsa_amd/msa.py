@@ -516,7 +516,21 @@ class MSAGPU:
         # spectrum is exactly zero and the kernels skip the work
         cols = np.nonzero(self.mask_device.cpu().numpy().any(axis=0))[0]
         self.band = (int(cols[0]), int(cols[-1]) + 1) if len(cols) else (0, Nx)
+        # where the CBED slice loop keeps the wave in 2 x 4 tiles, a tiled
+        # copy of the propagator lets the column pass read it the same way
+        self.pm_tiled_device = None
+        if _lib.load().namsa_wave_tiled(Ny, Nx):
+            self.pm_tiled_device = torch.empty_like(self.pm_device)
+            call("namsa_tile_plane", ptr(self.pm_device), ptr(self.pm_tiled_device), Ny, Nx, stream_handle())
         return self.pm_device
+
+    def cbed_propagator(self):
+        """(propagator pointer, mode flags) for namsa_multislice in CBED mode:
+        the tiled copy with NAMSA_PM_TILED where the slice loop tiles the
+        wave, else the row-major propagator."""
+        if getattr(self, "pm_tiled_device", None) is not None:
+            return ptr(self.pm_tiled_device), _lib.PM_TILED
+        return ptr(self.pm_device), 0
 
     def multislice(self, bandwidth=1 / 3, unified_mem=False, batch_size=None, transmit=True,
                    detectors=None, output="auto", group=None, moments=False, sink=None, records=False):
@@ -635,9 +649,10 @@ class MSAGPU:
                      np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
                      group, scale, ws_ptr, cbed_ptr, *det_args, ptr(mom_ws), ptr(self._moments_dev[start]), st)
             else:
+                pm_arg, flags = self.cbed_propagator() if mode == _lib.OUT_CBED else (ptr(self.pm_device), 0)
                 call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
-                     np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
-                     group, mode, scale, ws_ptr, cbed_ptr, *det_args, st)
+                     np.float32(self.kmax), t_arg, S, pm_arg, self.band[0], self.band[1],
+                     group, mode | flags, scale, ws_ptr, cbed_ptr, *det_args, st)
             if not resident:
                 streamer.push(dst, start, B, slot)
         if resident:

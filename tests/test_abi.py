@@ -56,6 +56,12 @@ def test_argument_errors_report_without_gpu():
     assert rc == -2 and b"tiled" in lib.namsa_last_error()
     rc = lib.namsa_slice_cols_tiled(ctypes.c_void_p(16), 1, 512, 512, ctypes.c_void_p(16), 0, 512, None)
     assert rc == -2 and b"tiled" in lib.namsa_last_error()
+    assert lib.namsa_tile_plane(ctypes.c_void_p(16), ctypes.c_void_p(16), 512, 512, None) == -1
+    assert lib.namsa_tile_plane(ctypes.c_void_p(16), ctypes.c_void_p(32), 511, 512, None) == -1
+    fake = ctypes.c_void_p(16)
+    rc = lib.namsa_multislice(fake, fake, 1, 512, 512, 8.0, fake, 1, fake, 0, 512, 1, L.OUT_CBED | L.PM_TILED, 1.0,
+                              fake, fake, None, 0, None, None, None)
+    assert rc == -1 and b"NAMSA_PM_TILED" in lib.namsa_last_error()
     # sized for the narrowest column tile of any schedule (4 columns)
     assert lib.namsa_det_workspace_doubles(4, 512, 512, 2) == 4 * (512 // 4) * 2
     assert lib.namsa_det_workspace_doubles(4, 500, 512, 2) == 0

@@ -144,11 +144,12 @@ def test_tiled_slice_passes_match_row_major(gpu, precise):
             autotune.select(N, 1, lib.namsa_sched_precise(N, 1, keep[1]))
         names = (autotune.schedules(N, 0)[autotune.selected(N, 0)], autotune.schedules(N, 1)[autotune.selected(N, 1)])
         print(f"N={N} B={B} schedules {names}", flush=True)
-        td, pmd = torch.from_numpy(t).to(gpu), torch.from_numpy(pm).to(gpu)
+        td = torch.from_numpy(t).to(gpu)
         st = L.stream_handle()
         out = []
-        for sfx, a in (("", psi), ("_tiled", _tile(psi))):
+        for sfx, a, p in (("", psi, pm), ("_tiled", _tile(psi), _tile(pm[None])[0])):
             w = torch.from_numpy(a).to(gpu)
+            pmd = torch.from_numpy(p).to(gpu)
             for band in ((0, N), (lo, hi)):
                 L.call("namsa_slice_rows" + sfx, L.ptr(w), B, N, N, L.ptr(td), band[0], band[1], st)
                 L.call("namsa_slice_cols" + sfx, L.ptr(w), B, N, N, L.ptr(pmd), lo, hi, st)
@@ -159,3 +160,37 @@ def test_tiled_slice_passes_match_row_major(gpu, precise):
     rm, tl = out[0], _untile(out[1])
     assert np.isfinite(rm).all()
     assert np.array_equal(rm.view(np.uint64), tl.view(np.uint64))
+
+
+def test_tile_plane_and_tiled_propagator(gpu):
+    """namsa_tile_plane writes the 2 x 4 tile layout, and a 2048^2 CBED run
+    given the tiled propagator (NAMSA_PM_TILED) reads the same values as one
+    given the row-major propagator: identical bits."""
+    import torch
+
+    N, B, S = 2048, 2, 3
+    rng = np.random.default_rng(11)
+    pm = (np.exp(1j * rng.uniform(0, 2 * np.pi, (N, N))) * (rng.random((N, N)) < 0.9)).astype(np.complex64)
+    lo, hi = autotune.default_band(N)
+    pm[:, :lo] = 0
+    pm[:, hi:] = 0
+    src = torch.from_numpy(pm).to(gpu)
+    dst = torch.empty_like(src)
+    st = L.stream_handle()
+    print(f"tile_plane {N}x{N}", flush=True)
+    L.call("namsa_tile_plane", L.ptr(src), L.ptr(dst), N, N, st)
+    assert np.array_equal(dst.cpu().numpy(), _tile(pm[None])[0])
+    psi_k = torch.from_numpy(((rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N)
+                             .astype(np.complex64)).to(gpu)
+    pos = torch.from_numpy(rng.uniform(-5, 5, (B, 2)).astype(np.float32)).to(gpu)
+    t = torch.from_numpy(np.exp(1j * rng.uniform(0, 2 * np.pi, (S, N, N))).astype(np.complex64)).to(gpu)
+    out = []
+    for p, flag in ((src, 0), (dst, L.PM_TILED)):
+        print(f"multislice {N}x{N} B={B} S={S} flags={flag}", flush=True)
+        ws = torch.empty((B, N, N), dtype=torch.complex64, device=gpu)
+        cb = torch.empty((B, N, N), dtype=torch.float32, device=gpu)
+        L.call("namsa_multislice", L.ptr(psi_k), L.ptr(pos), B, N, N, np.float32(6.5), L.ptr(t), S, L.ptr(p), lo, hi,
+               B, L.OUT_CBED | flag, np.float32(1.0), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
+        out.append(cb.cpu().numpy())
+    assert np.isfinite(out[0]).all() and out[0].max() > 0
+    assert np.array_equal(out[0], out[1])
