@@ -35,11 +35,12 @@ def _reference(N, E, radices, compact):
     ns = 1
     for R in radices:
         if ns > 1:
-            P = (4 if R >= 16 else 3 if R >= 8 else 1) if compact else R - 1
+            # compact: the powers of two of w up to w^(R-1) for R > 4, else w alone
+            P = (4 if R > 8 else 3 if R > 4 else 1) if compact else R - 1
             rows = []
             for q in range(E // R):
                 for r in range(1, P + 1):
-                    p = (1 << (r - 1)) if compact and R >= 8 else r
+                    p = (1 << (r - 1)) if compact and R > 4 else r
                     k = (np.arange(T) + T * q) % ns
                     turns = ((-p * k) % (ns * R)) / (ns * R)
                     rows.append([_unit_pair(np.cos(2 * np.pi * f), np.sin(2 * np.pi * f)) for f in turns])
@@ -69,7 +70,7 @@ def tables(tmp_path_factory):
 
 
 def test_tables_match_numpy(tables):
-    assert len(tables) == 8
+    assert len(tables) == 12
     for (N, E, radices, compact), got in tables.items():
         ref = _reference(N, E, radices, compact)
         assert got.shape == ref.shape, (N, radices, compact)

@@ -22,5 +22,10 @@ int main() {
   dump<Fft<1024, 16, Radices<16, 8, 8>, true>>("1024 16 16,8,8 1");
   dump<Fft<2048, 16, Radices<16, 16, 8>, false>>("2048 16 16,16,8 0");
   dump<Fft<2048, 16, Radices<16, 16, 8>, true>>("2048 16 16,16,8 1");
+  dump<Fft<768, 12, Radices<12, 4, 4, 4>, false>>("768 12 12,4,4,4 0");
+  dump<Fft<384, 12, Radices<12, 4, 4, 2>, true>>("384 12 12,4,4,2 1");
+  // radix 12 and 6 at a twiddled stage (compact: w, w^2, w^4 and w^8 stored)
+  dump<Fft<768, 12, Radices<4, 4, 4, 12>, true>>("768 12 4,4,4,12 1");
+  dump<Fft<384, 12, Radices<4, 4, 4, 6>, true>>("384 12 4,4,4,6 1");
   return 0;
 }
