@@ -484,11 +484,25 @@ class SpectralMaterial:
         B = pos.shape[0]
         G = min(B, int(group)) if group else B
         t, pm = self._slices(factor)
+        pm_arg, flags = self._cbed_pm(factor, pm)
         ws = torch.empty((G, self.Ny, self.Nx), dtype=torch.complex64, device=out.device)
         call("namsa_multislice", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
-             np.float32(self.sim.kmax), ptr(t), int(S), ptr(pm), self.band[0], self.band[1], G, _lib.OUT_CBED,
-             np.float32(1.0 / self.norm), ptr(ws), ptr(out), None, 0, None, None, stream_handle())
+             np.float32(self.sim.kmax), ptr(t), int(S), pm_arg, self.band[0], self.band[1], G,
+             _lib.OUT_CBED | flags, np.float32(1.0 / self.norm), ptr(ws), ptr(out), None, 0, None, None,
+             stream_handle())
         return out
+
+    def _cbed_pm(self, factor, pm):
+        """The propagator argument and mode flags of a CBED run: where the
+        slice loop tiles the wave, a tiled copy (made once per factor)."""
+        if not _lib.load().namsa_wave_tiled(self.Ny, self.Nx):
+            return ptr(pm), 0
+        key = 1 if factor in (None, 1) else factor
+        cache = self.__dict__.setdefault("_pm_tiled", {})
+        if key not in cache:
+            cache[key] = _torch().empty_like(pm)
+            call("namsa_tile_plane", ptr(pm), ptr(cache[key]), self.Ny, self.Nx, stream_handle())
+        return ptr(cache[key]), _lib.PM_TILED
 
     def vacuum_spectra(self, pos, thickness, out):
         """Exit spectra of the same probes through vacuum of `thickness`: one
