@@ -35,8 +35,8 @@ extern "C" {
 #define NAMSA_OUT_CBED 0     /* transmit=True: float32 CBED intensities   */
 #define NAMSA_OUT_INCIDENT 1 /* transmit=False: complex incident probes   */
 #define NAMSA_OUT_EXIT 2     /* complex real-space exit waves              */
-/* Or-ed into namsa_multislice's mode with NAMSA_OUT_CBED where
- * namsa_wave_tiled(Ny, Nx): pm is given in the 2 x 4 tile layout
+/* Or-ed into namsa_multislice's mode with NAMSA_OUT_CBED or NAMSA_OUT_EXIT
+ * where namsa_wave_tiled(Ny, Nx): pm is given in the 2 x 4 tile layout
  * (namsa_tile_plane), which the tiled column pass then reads in 64-byte
  * pieces too.  No counterpart in the reference. */
 #define NAMSA_PM_TILED 0x100
@@ -213,11 +213,13 @@ int namsa_slice_cols(void* psi_ws, int B, int Ny, int Nx, const void* pm, int ba
 /*
  * The same two passes over a wave stored in 2 x 4 tiles: element (y, x) of a
  * [Ny][Nx] plane at ((y/2)(Nx/4) + x/4)*8 + (y%2)*4 + x%4 (the column pass
- * takes pm in the same tiles; t_slice stays row-major).  In CBED mode
- * namsa_multislice keeps its internal wave in this layout at the sizes where
- * namsa_wave_tiled(Ny, Nx) returns 1 (2048 x 2048): the column pass then
- * reads 64-byte pieces instead of 32-byte row segments.  The tiled entry
- * points return NAMSA_ERR_UNSUPPORTED at other sizes.  No counterpart in the
+ * takes pm in the same tiles; t_slice stays row-major).  In CBED and
+ * exit-wave mode namsa_multislice keeps its wave in this layout at the sizes
+ * where namsa_wave_tiled(Ny, Nx) returns 1 (2048 x 2048): the column pass
+ * then reads 64-byte pieces instead of 32-byte row segments, and in
+ * exit-wave mode the last pass writes the exit waves back row-major in
+ * place.  The tiled entry points return NAMSA_ERR_UNSUPPORTED at other
+ * sizes.  No counterpart in the
  * reference (its cuFFT plans own their layouts); for per-pass timing and
  * custom slice loops at those sizes.
  */

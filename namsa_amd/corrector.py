@@ -470,10 +470,11 @@ class SpectralMaterial:
         the CBED."""
         B = pos.shape[0]
         t, pm = self._slices(factor)
+        pm_arg, flags = self._cbed_pm(factor, pm)
         scale = np.float32(1.0 / math.sqrt(self.Ny * self.Nx * self.norm))
         call("namsa_multislice", ptr(self.sim.psi_k_device), ptr(pos), B, self.Ny, self.Nx,
-             np.float32(self.sim.kmax), ptr(t), int(S), ptr(pm), self.band[0], self.band[1], B, _lib.OUT_EXIT,
-             scale, ptr(out), None, None, 0, None, None, stream_handle())
+             np.float32(self.sim.kmax), ptr(t), int(S), pm_arg, self.band[0], self.band[1], B,
+             _lib.OUT_EXIT | flags, scale, ptr(out), None, None, 0, None, None, stream_handle())
         call("namsa_fft2d", ptr(out), B, self.Ny, self.Nx, 0, stream_handle())
         return out
 
@@ -493,8 +494,9 @@ class SpectralMaterial:
         return out
 
     def _cbed_pm(self, factor, pm):
-        """The propagator argument and mode flags of a CBED run: where the
-        slice loop tiles the wave, a tiled copy (made once per factor)."""
+        """The propagator argument and mode flags of a CBED or exit-wave run:
+        where the slice loop tiles the wave, a tiled copy (made once per
+        factor)."""
         if not _lib.load().namsa_wave_tiled(self.Ny, self.Nx):
             return ptr(pm), 0
         key = 1 if factor in (None, 1) else factor

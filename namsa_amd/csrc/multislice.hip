@@ -419,12 +419,13 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
   const long long plane = static_cast<long long>(Ny) * Nx;
   const float inv_n2 = 1.0f / static_cast<float>(plane);
   const int G = (group > 0 && group < B) ? group : B;
-  // In CBED mode the wave never leaves the library, so where it pays the
-  // slice loop keeps it in the tiled layout (passes.hpp).
-  const bool tiled = mode == NAMSA_OUT_CBED && tiled_layout(Nx) && tiled_layout(Ny);
+  // Where it pays, the slice loop keeps the wave in the tiled layout
+  // (passes.hpp): in CBED mode it never leaves the library, and in exit-wave
+  // mode the last (inverse row) pass writes it back row-major in place.
+  const bool tiled = (mode == NAMSA_OUT_CBED || mode == NAMSA_OUT_EXIT) && tiled_layout(Nx) && tiled_layout(Ny);
   if (pm_tiled && !tiled)
-    return fail(kErrArg, "namsa_multislice: NAMSA_PM_TILED needs NAMSA_OUT_CBED at a tiled size (not %d x %d)",
-                Ny, Nx);
+    return fail(kErrArg, "namsa_multislice: NAMSA_PM_TILED needs CBED or exit-wave mode at a tiled size (not "
+                "%d x %d)", Ny, Nx);
 
   for (int g0 = 0; g0 < B; g0 += G) {
     const int Bg = std::min(G, B - g0);
@@ -471,8 +472,9 @@ int multislice_impl(const void* psi_k, const float* pos_yx, int B, int Ny, int N
     }
     if (mode == NAMSA_OUT_EXIT) {
       const int lo = S == 0 ? 0 : band_lo, hi = S == 0 ? Nx : band_hi;
-      if (int rc = row_pass(Nx, ROW_INV, row_args(psi, Bg, Ny, Nx, nullptr, out_scale, lo, hi), st))
-        return rc;
+      RowArgs r = row_args(psi, Bg, Ny, Nx, nullptr, out_scale, lo, hi);
+      r.tiled = a.tiled;  // reads tiles, writes rows
+      if (int rc = row_pass(Nx, ROW_INV, r, st)) return rc;
       continue;
     }
     // CBED readout.  S > 0: the state is u = R(t x) (no 1/N^2 on the row

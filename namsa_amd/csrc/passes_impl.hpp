@@ -131,7 +131,10 @@ __device__ __forceinline__ float2 unit_phase(double turns) {
 // ------------------------------------------------------------------ row pass
 // TL: the wave is stored in 2 x 4 tiles (tile_offset); rows r = 2k, 2k + 1 of
 // the stacked images form tile row k, and element t + T m of a row sits at a
-// lane base plus 2 T m.  The two rows of a workgroup are interleaved over the
+// lane base plus 2 T m.  The inverse row transform (ROW_INV, the exit wave)
+// reads tiles and writes rows: tile row k and rows 2k, 2k + 1 occupy the same
+// 2 Nx elements, which one workgroup holds in registers, so the conversion
+// runs in place.  The two rows of a workgroup are interleaved over the
 // lanes (row g = lane % 2), so that each wavefront access covers whole
 // tiles: with device-scope stores, half-tile pieces written by two
 // wavefronts took 40% longer than whole tiles (2048^2, 6 images: a
@@ -179,8 +182,11 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
     if (trow >= rows_img) trow -= rows_img;
   };
   constexpr int XS = TL ? 2 * T : T;  // element stride of a row in the wave layout
+  constexpr bool TL_OUT = TL && OP != ROW_INV;  // stores in the tiled layout too
+  constexpr int XS_OUT = TL_OUT ? 2 * T : T;
   static_assert(!TL || T % 4 == 0, "tiled rows need whole tiles per thread group");
   auto row_base = [&](long long r) { return TL ? a.psi + tile_offset(r, t, N) : a.psi + r * N + t; };
+  auto out_base = [&](long long r) { return TL_OUT ? a.psi + tile_offset(r, t, N) : a.psi + r * N + t; };
   auto load = [&](long long r, float2(&v)[E], float2(&tv)[E]) {
     const float2* p = row_base(r);
 #pragma unroll
@@ -229,11 +235,11 @@ __global__ void __launch_bounds__(kRowThreads, S::W > 0 ? S::W : 1) row_kernel(R
       F::template run<false, Sync>(v, lds, t, tw);
     }
     if (row < a.n_rows) {
-      float2* p = row_base(row);
+      float2* p = out_base(row);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int x = t + T * m;
-        if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + XS * m, v[m]);
+        if (x >= a.store_lo && x < a.store_hi) st<S::NT>(p + XS_OUT * m, v[m]);
       }
     }
   }
@@ -510,7 +516,7 @@ int persistent_cap(K kernel, int threads) {
 
 template <class S, int OP, bool TL = false>
 int launch_row_op(const RowArgs& a, hipStream_t st) {
-  if constexpr (!TL && OP == ROW_STEP && tiled_layout(S::N)) {
+  if constexpr (!TL && (OP == ROW_STEP || OP == ROW_INV) && tiled_layout(S::N)) {
     if (a.tiled) return launch_row_op<S, OP, true>(a, st);
   }
   if (!TL && a.tiled) return fail(kErrArg, "row pass: no tiled-layout form of this op at N = %d", S::N);

@@ -525,9 +525,9 @@ class MSAGPU:
         return self.pm_device
 
     def cbed_propagator(self):
-        """(propagator pointer, mode flags) for namsa_multislice in CBED mode:
-        the tiled copy with NAMSA_PM_TILED where the slice loop tiles the
-        wave, else the row-major propagator."""
+        """(propagator pointer, mode flags) for namsa_multislice in CBED or
+        exit-wave mode: the tiled copy with NAMSA_PM_TILED where the slice
+        loop tiles the wave, else the row-major propagator."""
         if getattr(self, "pm_tiled_device", None) is not None:
             return ptr(self.pm_tiled_device), _lib.PM_TILED
         return ptr(self.pm_device), 0
@@ -649,7 +649,8 @@ class MSAGPU:
                      np.float32(self.kmax), t_arg, S, ptr(self.pm_device), self.band[0], self.band[1],
                      group, scale, ws_ptr, cbed_ptr, *det_args, ptr(mom_ws), ptr(self._moments_dev[start]), st)
             else:
-                pm_arg, flags = self.cbed_propagator() if mode == _lib.OUT_CBED else (ptr(self.pm_device), 0)
+                pm_arg, flags = (self.cbed_propagator() if mode in (_lib.OUT_CBED, _lib.OUT_EXIT)
+                                 else (ptr(self.pm_device), 0))
                 call("namsa_multislice", ptr(self.psi_k_device), ptr(pos_all[start]), B, Ny, Nx,
                      np.float32(self.kmax), t_arg, S, pm_arg, self.band[0], self.band[1],
                      group, mode | flags, scale, ws_ptr, cbed_ptr, *det_args, st)

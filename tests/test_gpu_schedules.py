@@ -163,9 +163,9 @@ def test_tiled_slice_passes_match_row_major(gpu, precise):
 
 
 def test_tile_plane_and_tiled_propagator(gpu):
-    """namsa_tile_plane writes the 2 x 4 tile layout, and a 2048^2 CBED run
-    given the tiled propagator (NAMSA_PM_TILED) reads the same values as one
-    given the row-major propagator: identical bits."""
+    """namsa_tile_plane writes the 2 x 4 tile layout, and 2048^2 CBED and
+    exit-wave runs given the tiled propagator (NAMSA_PM_TILED) read the same
+    values as runs given the row-major propagator: identical bits."""
     import torch
 
     N, B, S = 2048, 2, 3
@@ -184,13 +184,14 @@ def test_tile_plane_and_tiled_propagator(gpu):
                              .astype(np.complex64)).to(gpu)
     pos = torch.from_numpy(rng.uniform(-5, 5, (B, 2)).astype(np.float32)).to(gpu)
     t = torch.from_numpy(np.exp(1j * rng.uniform(0, 2 * np.pi, (S, N, N))).astype(np.complex64)).to(gpu)
-    out = []
-    for p, flag in ((src, 0), (dst, L.PM_TILED)):
-        print(f"multislice {N}x{N} B={B} S={S} flags={flag}", flush=True)
-        ws = torch.empty((B, N, N), dtype=torch.complex64, device=gpu)
-        cb = torch.empty((B, N, N), dtype=torch.float32, device=gpu)
-        L.call("namsa_multislice", L.ptr(psi_k), L.ptr(pos), B, N, N, np.float32(6.5), L.ptr(t), S, L.ptr(p), lo, hi,
-               B, L.OUT_CBED | flag, np.float32(1.0), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
-        out.append(cb.cpu().numpy())
-    assert np.isfinite(out[0]).all() and out[0].max() > 0
-    assert np.array_equal(out[0], out[1])
+    for mode in (L.OUT_CBED, L.OUT_EXIT):
+        out = []
+        for p, flag in ((src, 0), (dst, L.PM_TILED)):
+            print(f"multislice {N}x{N} B={B} S={S} mode={mode} flags={flag}", flush=True)
+            ws = torch.empty((B, N, N), dtype=torch.complex64, device=gpu)
+            cb = torch.empty((B, N, N), dtype=torch.float32, device=gpu)
+            L.call("namsa_multislice", L.ptr(psi_k), L.ptr(pos), B, N, N, np.float32(6.5), L.ptr(t), S, L.ptr(p),
+                   lo, hi, B, mode | flag, np.float32(1.0), L.ptr(ws), L.ptr(cb), None, 0, None, None, st)
+            out.append((cb if mode == L.OUT_CBED else ws).cpu().numpy())
+        assert np.isfinite(out[0]).all() and np.abs(out[0]).max() > 0
+        assert np.array_equal(out[0], out[1])
