@@ -117,3 +117,48 @@ def test_noalloc_rejects_groups_beyond_scratch(gpu):
                                       np.float32(1.0), None, None, 2, None, None)
     assert rc == -1
     assert b"exceeds" in lib.namsa_last_error()
+
+
+def test_noalloc_tiled_2048_matches_multislice(gpu):
+    """At 2048^2 both entry points keep the wave in 2 x 4 tiles (one probe per
+    group in the 64 MiB scratch); namsa_multislice given the tiled propagator
+    (NAMSA_PM_TILED) and the allocation-free call given the row-major one
+    read the same values: identical CBEDs and detector signals."""
+    import torch
+
+    N, B, S = 2048, 2, 3
+    assert _capacity(N, 1) == 1
+    rng = np.random.default_rng(29)
+    st = L.stream_handle()
+    c64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex64)).to(gpu)
+    psi_k = c64((rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / N)
+    pos = torch.from_numpy(rng.uniform(-5, 5, (B, 2)).astype(np.float32)).to(gpu)
+    t = c64(np.exp(1j * rng.uniform(0, 2 * np.pi, (S, N, N))))
+    lo, hi = N // 6, N - N // 6
+    pm = np.exp(1j * rng.uniform(0, 2 * np.pi, (N, N)))
+    pm[:, :lo] = 0
+    pm[:, hi:] = 0
+    pm = c64(pm)
+    pm_t = torch.empty_like(pm)
+    L.call("namsa_tile_plane", L.ptr(pm), L.ptr(pm_t), N, N, st)
+    det = torch.from_numpy(rng.random((1, N, N)).astype(np.float32)).to(gpu)
+    out = []
+    for noalloc in (False, True):
+        _say(f"2048^2 B={B} S={S} noalloc={noalloc}")
+        cbed = torch.full((B, N, N), np.nan, dtype=torch.float32, device=gpu)
+        det_out = torch.full((B, 1), np.nan, dtype=torch.float64, device=gpu)
+        if noalloc:
+            L.call("namsa_multislice_noalloc", L.ptr(psi_k), L.ptr(pos), B, N, N, np.float32(6.5), L.ptr(t), S,
+                   L.ptr(pm), lo, hi, 0, np.float32(1.0), L.ptr(cbed), L.ptr(det), 1, L.ptr(det_out), st)
+        else:
+            ws = torch.empty((1, N, N), dtype=torch.complex64, device=gpu)
+            det_ws = torch.empty(int(L.load().namsa_det_workspace_doubles(1, N, N, 1)), dtype=torch.float64,
+                                 device=gpu)
+            L.call("namsa_multislice", L.ptr(psi_k), L.ptr(pos), B, N, N, np.float32(6.5), L.ptr(t), S, L.ptr(pm_t),
+                   lo, hi, 1, L.OUT_CBED | L.PM_TILED, np.float32(1.0), L.ptr(ws), L.ptr(cbed), L.ptr(det), 1,
+                   L.ptr(det_ws), L.ptr(det_out), st)
+        torch.cuda.synchronize()
+        out.append((cbed.cpu().numpy(), det_out.cpu().numpy()))
+    (c0, d0), (c1, d1) = out
+    assert np.isfinite(c0).all() and c0.max() > 0
+    assert np.array_equal(c0, c1) and np.array_equal(d0, d1)
