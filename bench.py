@@ -56,7 +56,8 @@ def parse():
     p.add_argument("--c5-probes", type=int, default=256,
                    help="config5: raster points timed (evenly spaced over the 64x64 raster; 4096 = all)")
     p.add_argument("--c5-chunk", type=int, default=128, help="config5: positions per streamed chunk")
-    p.add_argument("--c5-learned-chunk", type=int, default=4, help="config5-learned: probes per corrector call")
+    p.add_argument("--c5-learned-chunk", type=int, default=6,
+                   help="config5-learned: probes per coarse run and corrector call (6: the 2048^2 probe group)")
     p.add_argument("--c4-structures", type=int, default=32, help="config4: held-out structures scored (4 probes each)")
     p.add_argument("--streams", type=int, default=4, help="config3: streams the batch is dealt over")
     p.add_argument("--tune", action="store_true",
