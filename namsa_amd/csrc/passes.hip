@@ -55,7 +55,7 @@ static const SchedTable* table_for(int n) {
 // step there, slower at smaller sizes, where the slice rows stay cached).
 //                                       128 192 256 384 512 768 1024 2048
 static const int kDefaultRow[kNumSizes] = { 1,  2,  0,  3,  6,  3,  3,  4};
-static const int kDefaultCol[kNumSizes] = { 2,  2,  2,  7, 15,  9,  6,  6};
+static const int kDefaultCol[kNumSizes] = { 2,  2,  1,  7, 15,  9,  6,  6};
 
 static std::atomic<int> g_row_sel[kNumSizes];
 static std::atomic<int> g_col_sel[kNumSizes];
