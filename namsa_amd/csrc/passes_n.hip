@@ -182,6 +182,7 @@ using R1 = RowS<1024, 16, Radices<16, 8, 8>, true, false>;
 using R2 = RowS<1024, 8, Radices<8, 8, 4, 4>, true, false>;
 using R3 = RowS<1024, 16, Radices<16, 8, 8>, false, false, true>;
 using R4 = RowS<1024, 16, Radices<16, 8, 8>, false, false, false, 3>;
+using R5 = RowS<1024, 16, Radices<16, 8, 8>, false, false, false, 0, false, true>;
 using C0 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false>;
 using C1 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false>;
 using C2 = ColS<1024, 8, Radices<8, 8, 4, 4>, 8, false, false>;
@@ -189,7 +190,8 @@ static const RowSchedEntry kRows[] = {{"e16.r16.8.8", launch_row<R0, R0>},
                                       {"e16.r16.8.8.pf", launch_row<R1, R0>},
                                       {"e8.r8844.pf", launch_row<R2, R0>},
                                       {"e16.r16.8.8.ct", launch_row<R3, R0>},
-                                      {"e16.r16.8.8.w3", launch_row<R4, R0>}};
+                                      {"e16.r16.8.8.w3", launch_row<R4, R0>},
+                                      {"e16.r16.8.8.ls", launch_row<R5, R0>}};
 using C3 = ColS<1024, 16, Radices<16, 8, 8>, 8, true, false, false, true>;
 using C4 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, false, 4, true, true>;
 using C5 = ColS<1024, 16, Radices<16, 8, 8>, 8, false, false, false, true, 4, true, true>;
