@@ -37,12 +37,11 @@ static const SchedTable* table_for(int n) {
 // The row pass keeps its memory traffic under its arithmetic only with enough
 // wavefronts per SIMD: at 512^2 the schedule holding every twiddle reaches
 // that under a register budget (.w5), at 384^2 and 768^2 the radix-12 ones
-// holding every twiddle do on their own, at 1024^2 the one with split LDS
-// padding (.ls: 191 -> 161 VGPRs, 3% faster than the compact form), and at
-// 2048^2 only the compact-twiddle (.ct) one does (1-3% faster than the full
-// ones); the fused driver switches compact schedules to full ones for thick
-// runs (see precise_slices), where the compact forms' round-off would
-// approach 1e-4.
+// holding every twiddle do on their own, and at 1024^2 and 2048^2 the ones
+// with split LDS padding (.ls: 191 -> 161 and 190 -> 168 VGPRs), 1-3%
+// faster than the compact forms.  The fused driver switches compact
+// schedules to full ones for thick runs (see precise_slices), where the
+// compact forms' round-off would approach 1e-4.
 // The column pass runs several workgroups per CU (.w<k>: register budget,
 // one multi-workgroup launch of (strip, image) tasks; .sb: scalar-base
 // addressing) at 128^2 and 384^2-2048^2, 4-24% faster per slice step than the
@@ -56,7 +55,7 @@ static const SchedTable* table_for(int n) {
 // transmission row across the group's images (.tr; 5-8% faster per slice
 // step there, slower at smaller sizes, where the slice rows stay cached).
 //                                       128 192 256 384 512 768 1024 2048
-static const int kDefaultRow[kNumSizes] = { 1,  2,  0,  3,  6,  3,  5,  4};
+static const int kDefaultRow[kNumSizes] = { 1,  2,  0,  3,  6,  3,  5,  5};
 static const int kDefaultCol[kNumSizes] = { 2,  2,  1,  7, 15,  9,  6,  6};
 
 static std::atomic<int> g_row_sel[kNumSizes];
