@@ -77,10 +77,11 @@ constexpr int kMaxColTile = 16;
 // row steps followed by a band-limited column pass store this-aligned bands
 constexpr int kStoreAlign = kMaxColTile;
 // Thicker runs than this take the slice loop's passes from the schedules that
-// hold every twiddle instead of compact-twiddle ones.  The compact forms
-// accumulate about 9e-8 of the CBED per slice for the power-of-two lengths
-// and 2e-7 for the 4-5 stage radix-3 lengths (the full forms 4.5e-8 and
-// 1.2e-7), so the switch keeps every thickness below ~6e-5.
+// hold every twiddle instead of compact-twiddle ones.  With both passes
+// compact the CBED drifts by about 9e-8 per slice at the power-of-two
+// lengths and 1.3e-7-1.6e-7 at the lengths with a factor 3 (radix-12 first
+// stages; the full forms 4.5e-8 and 7e-8), so the switch keeps every
+// thickness below ~6e-5.
 constexpr int precise_slices(int n) { return n % 3 == 0 ? 200 : 300; }
 
 bool supported(int n);
